@@ -1,0 +1,297 @@
+/*
+ * lsamd.h -- C ABI of liblsamd.so, the MI355X (gfx950) engine for the
+ * lattice-symmetries hot path: symmetry-adapted basis construction and the
+ * sparse Hamiltonian product y = H x.
+ *
+ * Part 1 declares the reference's own entry points with the reference's
+ * names and signatures.  liblsamd.so exports all of them, so it replaces the
+ * reference's libkernels (kernels/reference.c, kernels/indexing.c,
+ * kernels/kernels.c) and the four table functions of the Chapel library
+ * (chapel/src/{StatesEnumeration,BatchedOperator,DistributedMatrixVector}.chpl)
+ * without any change to the callers.  Arguments are
+ * host pointers, as in the reference; the work runs on the GPU.
+ *
+ * Part 2 declares additive entry points (prefix ls_amd_) that keep data
+ * resident in device memory: basis build, index, CSR cache, FP64 / FP32 /
+ * block products and the Lanczos driver.  They have no reference
+ * counterpart; each says which reference routine it generalises.
+ *
+ * Errors follow the reference: invalid arguments go to ls_hs_error (a
+ * settable handler, abort by default); after the handler returns the entry
+ * point returns without touching its outputs further.
+ */
+#ifndef LSAMD_H
+#define LSAMD_H
+
+#include "lattice_symmetries_types.h"
+
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================== */
+/* Part 1: reference entry points                                            */
+/* ======================================================================== */
+
+/* --- errors, refcounts, external arrays: kernels/reference.c ------------- */
+
+/* reference.c:11-15 -- print and abort. */
+__attribute__((noreturn)) void ls_hs_fatal_error(char const *func, int line,
+                                                 char const *message);
+/* reference.c:25-30 -- NULL restores the default (print and abort). */
+void ls_hs_set_exception_handler(void (*handler)(char const *message));
+/* reference.c:32-36 */
+void ls_hs_error(char const *message);
+/* reference.c:40-48 */
+void ls_hs_internal_destroy_external_array(chpl_external_array *arr);
+/* reference.c:50-64 */
+int ls_hs_internal_read_refcount(int const *refcount);
+void ls_hs_internal_write_refcount(int *refcount, int value);
+int ls_hs_internal_inc_refcount(int *refcount);
+int ls_hs_internal_dec_refcount(int *refcount);
+
+/* --- term application: kernels/reference.c:67-134 ------------------------ */
+
+/* reference.c:67-95: ys[i] = sum over diagonal terms of Re(v) (-1)^.. xs[i]
+   (xs may be NULL: factor 1). */
+void ls_internal_operator_apply_diag_x1(ls_hs_operator const *op,
+                                        ptrdiff_t batch_size,
+                                        uint64_t const *alphas, double *ys,
+                                        double const *xs);
+/* reference.c:97-134: for each alpha, every matching off-diagonal term gives
+   one (beta, coeff); offsets[batch_size + 1] delimits the rows. */
+void ls_internal_operator_apply_off_diag_x1(ls_hs_operator const *op,
+                                            ptrdiff_t batch_size,
+                                            uint64_t const *alphas,
+                                            uint64_t *betas,
+                                            ls_hs_scalar *coeffs,
+                                            ptrdiff_t *offsets,
+                                            double const *xs);
+
+/* --- basis dispatchers: kernels/reference.c:137-211 ---------------------- */
+
+/* reference.c:137-148 */
+void ls_hs_state_index(ls_hs_basis const *basis, ptrdiff_t batch_size,
+                       uint64_t const *spins, ptrdiff_t spins_stride,
+                       ptrdiff_t *indices, ptrdiff_t indices_stride);
+/* reference.c:150-159 */
+void ls_hs_is_representative(ls_hs_basis const *basis, ptrdiff_t batch_size,
+                             uint64_t const *alphas, ptrdiff_t alphas_stride,
+                             uint8_t *are_representatives, double *norms);
+/* reference.c:161-171 */
+void ls_hs_state_info(ls_hs_basis const *basis, ptrdiff_t batch_size,
+                      uint64_t const *alphas, ptrdiff_t alphas_stride,
+                      uint64_t *betas, ptrdiff_t betas_stride,
+                      ls_hs_scalar *characters, double *norms);
+/* reference.c:173-194: enumerate through the registered engine table, then
+   build the state index with 22 prefix bits. */
+void ls_hs_build_representatives(ls_hs_basis *basis, uint64_t lower,
+                                 uint64_t upper);
+/* reference.c:196-211 */
+void ls_hs_unchecked_set_representatives(ls_hs_basis *basis,
+                                         chpl_external_array const *states,
+                                         int cache_bits);
+/* reference.c:214-225 */
+ls_chpl_kernels const *ls_hs_internal_get_chpl_kernels(void);
+void ls_hs_internal_set_chpl_kernels(ls_chpl_kernels const *kernels);
+
+/* --- state index: kernels/indexing.c ------------------------------------- */
+
+typedef struct ls_hs_state_index_binary_search_data
+    ls_hs_state_index_binary_search_data;
+
+/* indexing.c:94-117.  The representatives are borrowed (host memory, sorted
+   ascending); the engine also keeps a device copy plus a device prefix table
+   until the data are destroyed. */
+ls_hs_state_index_binary_search_data *
+ls_hs_create_state_index_binary_search_kernel_data(
+    chpl_external_array const *representatives, int number_bits,
+    int prefix_bits);
+/* indexing.c:119-125 */
+void ls_hs_destroy_state_index_binary_search_kernel_data(
+    ls_hs_state_index_binary_search_data *cache);
+/* indexing.c:273-325: index of each spin, -1 when absent; strides must be 1. */
+void ls_hs_state_index_binary_search_kernel(ptrdiff_t batch_size,
+                                            uint64_t const *spins,
+                                            ptrdiff_t spins_stride,
+                                            ptrdiff_t *indices,
+                                            ptrdiff_t indices_stride,
+                                            void const *private_kernel_data);
+
+/* --- orbit kernels: kernels/kernels.c + kernels/generator.cpp ------------ */
+
+typedef struct ls_internal_halide_kernel_data ls_internal_halide_kernel_data;
+
+/* kernels.c:112-189.  Copies the group (masks, shifts, characters) to the
+   device; the group may be destroyed afterwards.  spin_inversion must be
+   -1, 0 or 1. */
+ls_internal_halide_kernel_data *
+ls_internal_create_halide_kernel_data(ls_hs_permutation_group const *g,
+                                      int spin_inversion);
+/* kernels.c:191-193 */
+void ls_internal_destroy_halide_kernel_data(ls_internal_halide_kernel_data *p);
+/* kernels.c:195-244 + generator.cpp:168-272 */
+void ls_hs_is_representative_halide_kernel(ptrdiff_t batch_size,
+                                           uint64_t const *alphas,
+                                           ptrdiff_t alphas_stride,
+                                           uint8_t *are_representatives,
+                                           double *norms,
+                                           void const *private_data);
+/* kernels.c:246-319 + generator.cpp:10-166 */
+void ls_hs_state_info_halide_kernel(ptrdiff_t batch_size,
+                                    uint64_t const *alphas,
+                                    ptrdiff_t alphas_stride, uint64_t *betas,
+                                    ptrdiff_t betas_stride,
+                                    ls_hs_scalar *characters, double *norms,
+                                    void const *private_data);
+
+/* --- engine table functions: chapel/src/ --------------------------------- */
+
+/* chapel/src/library.c:19-34 and LatticeSymmetries.chpl:17-32: initialise
+   the device and register the four functions below in the global table. */
+void ls_chpl_init(void);
+void ls_chpl_finalize(void);
+/* StatesEnumeration.chpl:692-709.  Representatives of `p` in [lower, upper],
+   ascending; `dest` receives a malloc'ed array and its freer. */
+void ls_chpl_enumerate_representatives(ls_hs_basis const *p, uint64_t lower,
+                                       uint64_t upper,
+                                       chpl_external_array *dest);
+/* BatchedOperator.chpl:298-316 */
+void ls_chpl_operator_apply_diag(ls_hs_operator *matrixPtr, int64_t count,
+                                 uint64_t *alphas, chpl_external_array *coeffs,
+                                 int64_t numTasks);
+/* BatchedOperator.chpl:318-357 */
+void ls_chpl_operator_apply_off_diag(ls_hs_operator *matrixPtr, int64_t count,
+                                     uint64_t *alphas,
+                                     chpl_external_array *betas,
+                                     chpl_external_array *coeffs,
+                                     chpl_external_array *offsets,
+                                     int64_t numTasks);
+/* DistributedMatrixVector.chpl:1090-1105: y = H x for one real vector over
+   the representatives of op->basis (numVectors must be 1).  The diagonal
+   assigns y; without diagonal terms y is accumulated into (:1054-1057). */
+void ls_chpl_matrix_vector_product(ls_hs_operator *matrixPtr, int numVectors,
+                                   double const *xPtr, double *yPtr);
+
+/* ======================================================================== */
+/* Part 2: additive entry points (device-resident data)                      */
+/* ======================================================================== */
+
+/* Library status.  Returns 0 when a gfx950 device is usable, else nonzero. */
+int ls_amd_device_available(void);
+/* Last error message set by the engine (empty string when none). */
+char const *ls_amd_last_error(void);
+void ls_amd_clear_error(void);
+
+/* Device memory for callers without their own allocator. */
+void *ls_amd_device_malloc(size_t bytes);
+void ls_amd_device_free(void *ptr);
+int ls_amd_memcpy_h2d(void *dst, void const *src, size_t bytes);
+int ls_amd_memcpy_d2h(void *dst, void const *src, size_t bytes);
+int ls_amd_synchronize(void);
+
+/* Group set-up (Group.hs:106-132, Benes.hs:316-377 restated in C++): closure
+   of `number_generators` permutations of n sites (bit i of g(x) = bit
+   p[i] of x) with momentum sectors, as the reference's
+   ls_hs_permutation_group (ascending group order, characters
+   cos/sin(-2 pi phase), Benes networks of depth 2 log2(next pow2 >= n) - 1).
+   Returns NULL (and sets the last error) for inconsistent input. */
+ls_hs_permutation_group *ls_amd_permutation_group_create(
+    int number_sites, int number_generators, int32_t const *permutations,
+    int32_t const *sectors);
+void ls_amd_permutation_group_destroy(ls_hs_permutation_group *g);
+/* Group elements as permutations, [number_masks][number_sites]. */
+int ls_amd_permutation_group_elements(ls_hs_permutation_group const *g,
+                                      int32_t *out);
+
+/* Basis handle: a spin basis (or a basis without permutations) whose
+   representatives live in device memory.  hamming_weight = -1 for none.
+   group may be NULL (no permutations); spin_inversion in {-1, 0, 1}. */
+typedef struct ls_amd_basis ls_amd_basis;
+ls_amd_basis *ls_amd_basis_create(int number_sites, int hamming_weight,
+                                  int spin_inversion,
+                                  ls_hs_permutation_group const *group);
+void ls_amd_basis_destroy(ls_amd_basis *b);
+/* Basis.hs:734-774 (min/max state estimates). */
+uint64_t ls_amd_basis_min_state(ls_amd_basis const *b);
+uint64_t ls_amd_basis_max_state(ls_amd_basis const *b);
+/* GPU enumeration of representatives in [lower, upper] (ascending, as
+   StatesEnumeration.chpl:242-326); builds the device index as well.
+   Returns the number of states or -1. */
+int64_t ls_amd_basis_build(ls_amd_basis *b, uint64_t lower, uint64_t upper);
+int64_t ls_amd_basis_number_states(ls_amd_basis const *b);
+/* Number of candidate states examined by the last build. */
+int64_t ls_amd_basis_number_candidates(ls_amd_basis const *b);
+uint64_t const *ls_amd_basis_states_device(ls_amd_basis const *b);
+int ls_amd_basis_copy_states(ls_amd_basis const *b, uint64_t *host_out);
+/* Adopt a host list of sorted representatives instead of building. */
+int ls_amd_basis_set_states(ls_amd_basis *b, uint64_t const *host_states,
+                            int64_t count);
+/* Device versions of state_info / is_representative / index on device
+   arrays (strides 1). */
+int ls_amd_basis_state_info_device(ls_amd_basis const *b, int64_t count,
+                                   uint64_t const *d_alphas, uint64_t *d_betas,
+                                   double *d_characters, double *d_norms);
+int ls_amd_basis_is_representative_device(ls_amd_basis const *b, int64_t count,
+                                          uint64_t const *d_alphas,
+                                          uint8_t *d_flags, double *d_norms);
+int ls_amd_basis_index_device(ls_amd_basis const *b, int64_t count,
+                              uint64_t const *d_states, int64_t *d_indices);
+
+/* Operator handle over an ls_amd_basis: diagonal and off-diagonal
+   non-branching terms (one 64-bit word per mask, as the reference's
+   matvec requires, BatchedOperator.chpl:327-328). */
+typedef struct ls_amd_operator ls_amd_operator;
+ls_amd_operator *ls_amd_operator_create(ls_amd_basis const *b,
+                                        ls_hs_nonbranching_terms const *diag,
+                                        ls_hs_nonbranching_terms const *off_diag);
+void ls_amd_operator_destroy(ls_amd_operator *op);
+
+/* On-the-fly product in pull form (row i gathers from the representatives
+   of H|alpha_i>), FP64, real x / y on the device: the reference's
+   ls_chpl_matrix_vector_product with device pointers.  accumulate = 1 keeps
+   y as the base of the sum when there are no diagonal terms. */
+int ls_amd_matvec_f64(ls_amd_operator *op, double const *d_x, double *d_y,
+                      int accumulate);
+/* Complex x / y (interleaved re, im), y = M x with the full complex matrix
+   (the reference casts coefficients to real; this keeps them complex). */
+int ls_amd_matvec_c128(ls_amd_operator *op, double const *d_x, double *d_y);
+
+/* CSR cache: off-diagonal part as rows of (column, value) plus a dense
+   diagonal.  precision: 64 stores FP64 values, 32 stores FP32 values, 96
+   stores both.  Returns the number of stored off-diagonal nonzeros or -1. */
+int64_t ls_amd_operator_build_csr(ls_amd_operator *op, int precision);
+int64_t ls_amd_operator_csr_nnz(ls_amd_operator const *op);
+/* Stored CSR (off-diagonal) rows: row_ptr[n+1] (int64), cols, values. */
+int ls_amd_operator_copy_csr(ls_amd_operator const *op, int64_t *row_ptr,
+                             uint32_t *cols, double *values, double *diag);
+/* y = H x from the cached CSR: FP64 one vector. */
+int ls_amd_csr_matvec_f64(ls_amd_operator *op, double const *d_x, double *d_y);
+/* FP32 block of k right-hand sides, X and Y row-major [n][k] (k <= 16). */
+int ls_amd_csr_matvec_f32(ls_amd_operator *op, int k, float const *d_x,
+                          float *d_y);
+/* FP64 block of k right-hand sides, row-major [n][k] (k <= 16). */
+int ls_amd_csr_matvec_f64_block(ls_amd_operator *op, int k, double const *d_x,
+                                double *d_y);
+/* Choose the CSR kernel schedule (lanes per row) by timing the candidates
+   on the current matrix; returns the chosen lanes per row. */
+int ls_amd_operator_autotune(ls_amd_operator *op);
+
+/* Timing of the last launched kernel of the given family, in milliseconds
+   (HIP events on the engine's stream). family: 0 csr f64, 1 csr f32 block,
+   2 on-the-fly f64, 3 enumeration, 4 csr build, 5 csr f64 block. */
+double ls_amd_last_kernel_ms(int family);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#endif /* LSAMD_H */

@@ -1,0 +1,390 @@
+// enumerate.hip -- representatives of a basis on the GPU.
+//
+// Restates chapel/src/StatesEnumeration.chpl (_enumerateStatesProjected
+// :242-268, _enumerateStatesUnprojected :269-289, the spinful per-sector
+// product :290-326) as three passes:
+//
+//  1. count:  every lane owns a contiguous range of candidates by
+//     combinadics rank (StatesEnumeration.chpl:130-152 splits the same way),
+//     unranks its first state and steps with Gosper's rule (:29-32).  The
+//     lane tests one group element per loop iteration and moves on to its
+//     next candidate as soon as the current one is rejected (an image below
+//     x) or accepted (all elements seen), so no lane idles while its
+//     neighbours finish long orbits.  Group masks and Re(chi) are staged in
+//     LDS.  Accepted candidates set a bit in a per-lane bitmask.
+//  2. scan:   exclusive prefix sum of the per-lane counts.
+//  3. write:  every lane replays its range and writes accepted states at
+//     its offset, which keeps the global order ascending.
+//
+// Acceptance is flag && norm > 0 with the norm summed in the chunk order of
+// generator.cpp:224-253 (chunks of four, chunk sums added left to right,
+// tail elements one by one), so sectors whose characters are not +-1 keep
+// exactly the states the restated reference keeps.
+#include <algorithm>
+
+#include "orbit_device.hpp"
+
+namespace lsamd {
+
+struct EnumParams {
+  uint64_t rank0;   // rank of the first candidate (or the state itself)
+  int64_t total;    // number of candidates
+  int64_t per_lane; // candidates per lane, multiple of 32
+  int64_t lanes;
+  int hamming; // -1: consecutive integers
+  uint64_t const *binom;
+};
+
+__device__ __forceinline__ uint64_t enum_unrank(EnumParams const &p, uint64_t r) {
+  if (p.hamming < 0) return r;
+  uint64_t state = 0;
+  int c = 63;
+  for (int i = p.hamming; i > 0; --i) {
+    while (c >= i && p.binom[c * 65 + i] > r) --c;
+    state |= uint64_t{1} << c;
+    r -= p.binom[c * 65 + i];
+    --c;
+  }
+  return state;
+}
+
+__device__ __forceinline__ uint64_t enum_next(int hamming, uint64_t v) {
+  if (hamming < 0) return v + 1;
+  uint64_t const t = v | (v - 1);
+  return (t + 1) | (((~t & (t + 1)) - 1) >> (__builtin_ctzll(v) + 1));
+}
+
+template <typename W, int D, int SI, bool LDS>
+__global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView g,
+                                                         uint32_t *__restrict__ bits,
+                                                         uint32_t *__restrict__ counts) {
+  extern __shared__ uint64_t smem[];
+  int const G = g.size;
+  int const depth = D > 0 ? D : g.depth;
+  W *s_masks = reinterpret_cast<W *>(smem);
+  double *s_re = reinterpret_cast<double *>(
+      smem + (static_cast<size_t>(G) * depth * sizeof(W) + 7) / 8);
+  if constexpr (LDS) {
+    for (int i = threadIdx.x; i < G * depth; i += blockDim.x) s_masks[i] = static_cast<W>(g.masks[i]);
+    for (int i = threadIdx.x; i < G; i += blockDim.x) s_re[i] = g.eig_re[i];
+    __syncthreads();
+  }
+  int64_t const lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const first = lane * p.per_lane;
+  int64_t remaining = 0;
+  if (lane < p.lanes && first < p.total) remaining = min(p.per_lane, p.total - first);
+  W x = remaining > 0 ? static_cast<W>(enum_unrank(p, p.rank0 + static_cast<uint64_t>(first))) : W{0};
+  uint32_t *my_bits = bits + lane * (p.per_lane / 32);
+  W const flip = static_cast<W>(g.flip);
+  int const nch4 = (G / kIsRepresentativeLanes) * kIsRepresentativeLanes;
+  int e = 0;
+  int64_t pos = 0;
+  uint32_t word = 0, count = 0;
+  double total = 0.0, chunk = 0.0;
+
+  while (__any(remaining > 0)) {
+    if (remaining > 0) {
+      W y;
+      double re;
+      if constexpr (LDS) {
+        y = benes_apply_ptr<W, D>(s_masks + static_cast<size_t>(e) * depth, x, g.shifts, depth);
+        re = s_re[e];
+      } else {
+        y = group_apply<W, D>(g, e, x);
+        re = g.eig_re[e];
+      }
+      bool ok = y >= x;
+      double t = static_cast<double>(y == x) * re;
+      if constexpr (SI != 0) {
+        W const yf = static_cast<W>(y ^ flip);
+        ok = ok && (yf >= x);
+        double const a = static_cast<double>(y == x), b = static_cast<double>(yf == x);
+        t = (SI == 1 ? (a + b) : (a - b)) * re;
+      }
+      bool done = false, accept = false;
+      if (!ok) {
+        done = true;
+      } else {
+        if (e < nch4) {
+          chunk = chunk + t;
+          if ((e & 3) == 3) {
+            total = total + chunk;
+            chunk = 0.0;
+          }
+        } else {
+          total = total + t;
+        }
+        ++e;
+        if (e == G) {
+          done = true;
+          accept = total > 0;
+        }
+      }
+      if (done) {
+        if (accept) {
+          word |= 1u << (pos & 31);
+          ++count;
+        }
+        ++pos;
+        --remaining;
+        if ((pos & 31) == 0 || remaining == 0) {
+          my_bits[(pos - 1) >> 5] = word;
+          word = 0;
+        }
+        if (remaining > 0) x = static_cast<W>(enum_next(p.hamming, static_cast<uint64_t>(x)));
+        e = 0;
+        total = 0.0;
+        chunk = 0.0;
+      }
+    }
+  }
+  if (lane < p.lanes) counts[lane] = count;
+}
+
+// Replays the candidate ranges and writes accepted states (all candidates
+// when bits == nullptr) starting at offsets[lane] (lane * per_lane when
+// offsets == nullptr).
+__global__ __launch_bounds__(256) void enum_write_kernel(EnumParams p,
+                                                         uint32_t const *__restrict__ bits,
+                                                         uint64_t const *__restrict__ offsets,
+                                                         uint64_t *__restrict__ out) {
+  int64_t const lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (lane >= p.lanes) return;
+  int64_t const first = lane * p.per_lane;
+  if (first >= p.total) return;
+  int64_t const n = min(p.per_lane, p.total - first);
+  uint64_t x = enum_unrank(p, p.rank0 + static_cast<uint64_t>(first));
+  uint64_t o = offsets != nullptr ? offsets[lane] : static_cast<uint64_t>(first);
+  uint32_t const *my_bits = bits != nullptr ? bits + lane * (p.per_lane / 32) : nullptr;
+  uint32_t word = 0xffffffffu;
+  for (int64_t k = 0; k < n; ++k) {
+    if (my_bits != nullptr && (k & 31) == 0) word = my_bits[k >> 5];
+    if ((word >> (k & 31)) & 1u) out[o++] = x;
+    if (k + 1 < n) x = enum_next(p.hamming, x);
+  }
+}
+
+// Spinful fermions with fixed particle numbers per species
+// (StatesEnumeration.chpl:290-326): x = (xB << n) | xA, xB outer, xA inner.
+__global__ __launch_bounds__(256) void enum_spinful_kernel(uint64_t const *__restrict__ binom,
+                                                           int n, int up, int down,
+                                                           uint64_t rankA0, uint64_t countA,
+                                                           uint64_t rankB0, uint64_t total,
+                                                           uint64_t *__restrict__ out) {
+  uint64_t const k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  EnumParams pa{0, 0, 0, 0, up, binom}, pb{0, 0, 0, 0, down, binom};
+  uint64_t const xa = enum_unrank(pa, rankA0 + k % countA);
+  uint64_t const xb = enum_unrank(pb, rankB0 + k / countA);
+  out[k] = (xb << n) | xa;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of uint32 counts into uint64 offsets
+// ---------------------------------------------------------------------------
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *s, uint64_t &total) {
+  int const t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    uint64_t const add = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  total = s[kScanBlock - 1];
+  uint64_t const incl = s[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(uint32_t const *__restrict__ in,
+                                                                 int64_t n,
+                                                                 uint64_t *__restrict__ sums) {
+  __shared__ uint64_t s[kScanBlock];
+  int64_t const base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  uint64_t acc = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t const i = base + static_cast<int64_t>(threadIdx.x) * kScanItems + k;
+    if (i < n) acc += in[i];
+  }
+  uint64_t total;
+  (void)block_exclusive_scan(acc, s, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of `n` tile sums in place, total at sums[n].
+__global__ __launch_bounds__(kScanBlock) void scan_tiles_kernel(uint64_t *__restrict__ sums, int64_t n) {
+  __shared__ uint64_t s[kScanBlock];
+  uint64_t carry = 0;
+  for (int64_t base = 0; base < n; base += kScanBlock) {
+    int64_t const i = base + threadIdx.x;
+    uint64_t const v = i < n ? sums[i] : 0;
+    uint64_t total;
+    uint64_t const ex = block_exclusive_scan(v, s, total);
+    if (i < n) sums[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) sums[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(uint32_t const *__restrict__ in,
+                                                                int64_t n,
+                                                                uint64_t const *__restrict__ sums,
+                                                                uint64_t *__restrict__ out) {
+  __shared__ uint64_t s[kScanBlock];
+  int64_t const base = static_cast<int64_t>(blockIdx.x) * kScanTile +
+                       static_cast<int64_t>(threadIdx.x) * kScanItems;
+  uint64_t acc = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t const i = base + k;
+    if (i < n) acc += in[i];
+  }
+  uint64_t total;
+  uint64_t run = sums[blockIdx.x] + block_exclusive_scan(acc, s, total);
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t const i = base + k;
+    if (i < n) {
+      out[i] = run;
+      run += in[i];
+    }
+  }
+}
+
+// offsets[0 .. n) exclusive sums of counts; returns the total.
+bool exclusive_scan(uint32_t const *d_counts, int64_t n, uint64_t *d_offsets, uint64_t &total) {
+  int64_t const tiles = (n + kScanTile - 1) / kScanTile;
+  DeviceArray<uint64_t> sums;
+  if (!sums.allocate(static_cast<size_t>(tiles + 1))) return false;
+  scan_reduce_kernel<<<static_cast<unsigned>(tiles), kScanBlock, 0, stream()>>>(d_counts, n, sums.ptr);
+  LSAMD_HIP(hipGetLastError());
+  scan_tiles_kernel<<<1, kScanBlock, 0, stream()>>>(sums.ptr, tiles);
+  LSAMD_HIP(hipGetLastError());
+  scan_apply_kernel<<<static_cast<unsigned>(tiles), kScanBlock, 0, stream()>>>(d_counts, n, sums.ptr,
+                                                                              d_offsets);
+  LSAMD_HIP(hipGetLastError());
+  LSAMD_HIP(hipMemcpyAsync(&total, sums.ptr + tiles, sizeof(uint64_t), hipMemcpyDeviceToHost, stream()));
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Driver
+// ---------------------------------------------------------------------------
+
+namespace {
+// Candidates per lane: enough lanes to fill the chip several times over
+// (256 CUs x 32 waves x 64 lanes ~ 0.5 M), at least 64 per lane.
+int64_t lanes_for(int64_t total, int64_t &per_lane) {
+  int64_t const target_lanes = int64_t{1} << 21;
+  per_lane = (total + target_lanes - 1) / target_lanes;
+  per_lane = std::max<int64_t>(64, ((per_lane + 31) / 32) * 32);
+  return (total + per_lane - 1) / per_lane;
+}
+} // namespace
+
+bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t upper,
+                      DeviceArray<uint64_t> &out, int64_t &count, int64_t &candidates) {
+  count = 0;
+  candidates = 0;
+  if (!ensure_device()) return false;
+  uint64_t const *binom = binomials_device();
+  if (binom == nullptr) return false;
+  int const n = spec.number_bits;
+  uint64_t const mask = n >= 64 ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
+
+  if (spec.spinful_per_sector) {
+    int const ns = spec.number_sites;
+    uint64_t const smask = (uint64_t{1} << ns) - 1;
+    uint64_t const minA = lower & smask, maxA = upper & smask;
+    uint64_t const minB = (lower >> ns) & smask, maxB = (upper >> ns) & smask;
+    if (__builtin_popcountll(minA) != spec.number_up || __builtin_popcountll(maxA) != spec.number_up ||
+        __builtin_popcountll(minB) != spec.number_down || __builtin_popcountll(maxB) != spec.number_down)
+      return fail("invalid lower/upper for a spinful fermion basis");
+    uint64_t const rA0 = fixed_hamming_state_to_index(minA);
+    uint64_t const countA = fixed_hamming_state_to_index(maxA) - rA0 + 1;
+    uint64_t const rB0 = fixed_hamming_state_to_index(minB);
+    uint64_t const countB = fixed_hamming_state_to_index(maxB) - rB0 + 1;
+    uint64_t const total = countA * countB;
+    if (!out.allocate(static_cast<size_t>(total))) return false;
+    enum_spinful_kernel<<<grid_for(static_cast<int64_t>(total), 256), 256, 0, stream()>>>(
+        binom, ns, spec.number_up, spec.number_down, rA0, countA, rB0, total, out.ptr);
+    LSAMD_HIP(hipGetLastError());
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    count = candidates = static_cast<int64_t>(total);
+    return true;
+  }
+
+  bool const projected = spec.group != nullptr && spec.group->size > 0;
+  uint64_t hi = upper;
+  if (!projected && spec.spin_inversion != 0) hi = std::min(hi, hi ^ mask); // :276-281
+  if (lower > hi) {
+    out.reset();
+    return true;
+  }
+  EnumParams p{};
+  p.hamming = spec.hamming_weight;
+  p.binom = binom;
+  if (p.hamming >= 0) {
+    if (__builtin_popcountll(lower) != p.hamming || __builtin_popcountll(hi) != p.hamming)
+      return fail("lower and upper must have the basis Hamming weight");
+    p.rank0 = fixed_hamming_state_to_index(lower);
+    p.total = static_cast<int64_t>(fixed_hamming_state_to_index(hi) - p.rank0 + 1);
+  } else {
+    p.rank0 = lower;
+    p.total = static_cast<int64_t>(hi - lower + 1);
+  }
+  candidates = p.total;
+  p.lanes = lanes_for(p.total, p.per_lane);
+  int const blocks = grid_for(p.lanes, 256);
+
+  if (!projected) {
+    if (!out.allocate(static_cast<size_t>(p.total))) return false;
+    timer_begin(kTimerEnumerate);
+    enum_write_kernel<<<blocks, 256, 0, stream()>>>(p, nullptr, nullptr, out.ptr);
+    LSAMD_HIP(hipGetLastError());
+    timer_end(kTimerEnumerate);
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    count = p.total;
+    return true;
+  }
+
+  GroupData const &g = *spec.group;
+  DeviceArray<uint32_t> bits, counts;
+  DeviceArray<uint64_t> offsets;
+  if (!bits.allocate(static_cast<size_t>(p.lanes * (p.per_lane / 32)))) return false;
+  if (!counts.allocate(static_cast<size_t>(p.lanes))) return false;
+  if (!offsets.allocate(static_cast<size_t>(p.lanes))) return false;
+  GroupView const v = g.view();
+  timer_begin(kTimerEnumerate);
+  bool ok = dispatch_group(g, [&]<typename W, int D, int SI>() -> bool {
+    int const depth = D > 0 ? D : g.depth;
+    size_t const smem = ((static_cast<size_t>(g.size) * depth * sizeof(W) + 7) / 8) * 8 +
+                        static_cast<size_t>(g.size) * sizeof(double);
+    if (smem <= 64 * 1024) {
+      enum_count_kernel<W, D, SI, true><<<blocks, 256, smem, stream()>>>(p, v, bits.ptr, counts.ptr);
+    } else {
+      enum_count_kernel<W, D, SI, false><<<blocks, 256, 0, stream()>>>(p, v, bits.ptr, counts.ptr);
+    }
+    LSAMD_HIP(hipGetLastError());
+    return true;
+  });
+  if (!ok) return false;
+  uint64_t total = 0;
+  if (!exclusive_scan(counts.ptr, p.lanes, offsets.ptr, total)) return false;
+  if (!out.allocate(static_cast<size_t>(total))) return false;
+  if (total > 0) {
+    enum_write_kernel<<<blocks, 256, 0, stream()>>>(p, bits.ptr, offsets.ptr, out.ptr);
+    LSAMD_HIP(hipGetLastError());
+  }
+  timer_end(kTimerEnumerate);
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  count = static_cast<int64_t>(total);
+  return true;
+}
+
+} // namespace lsamd

@@ -1,0 +1,369 @@
+// group.cpp -- permutation groups, characters and Benes networks (host).
+//
+// Restates the set-up that the reference does in Haskell
+// (haskell/src/LatticeSymmetries/Group.hs, Benes.hs) so that the engine can
+// build its own ls_hs_permutation_group, and uploads a group to the device
+// for ls_internal_create_halide_kernel_data (kernels/kernels.c:112-189).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <set>
+#include <utility>
+
+#include "lsamd_internal.hpp"
+
+namespace lsamd {
+namespace {
+
+using Perm = std::vector<int32_t>;
+
+// Phase as a reduced fraction num/den in [0, 1).
+struct Phase {
+  int64_t num = 0, den = 1;
+  bool operator<(Phase const &o) const {
+    return static_cast<__int128>(num) * o.den < static_cast<__int128>(o.num) * den;
+  }
+  bool operator==(Phase const &o) const { return num == o.num && den == o.den; }
+};
+
+Phase make_phase(int64_t num, int64_t den) {
+  int64_t g = std::gcd(num, den);
+  if (g == 0) g = 1;
+  num /= g;
+  den /= g;
+  if (den < 0) {
+    num = -num;
+    den = -den;
+  }
+  num %= den; // phases live in [0, 1) (Group.hs:90-93)
+  if (num < 0) num += den;
+  if (num == 0) den = 1;
+  return Phase{num, den};
+}
+
+Phase add_phase(Phase a, Phase b) {
+  return make_phase(a.num * b.den + b.num * a.den, a.den * b.den);
+}
+
+// (a <> b)[i] = b[a[i]]: apply a's relabelling first (Benes.hs:100-101).
+Perm compose(Perm const &a, Perm const &b) {
+  Perm c(a.size());
+  for (size_t i = 0; i < a.size(); ++i) c[i] = b[static_cast<size_t>(a[i])];
+  return c;
+}
+
+bool is_identity(Perm const &p) {
+  for (size_t i = 0; i < p.size(); ++i)
+    if (p[i] != static_cast<int32_t>(i)) return false;
+  return true;
+}
+
+int order_of(Perm const &p) {
+  Perm cur = p;
+  int k = 1;
+  while (!is_identity(cur)) {
+    cur = compose(p, cur);
+    ++k;
+  }
+  return k;
+}
+
+// ---------------------------------------------------------------------------
+// Benes network routing.
+//
+// For N = 2^L positions, element q must travel to position dst[q].  The
+// network is: a delta-swap stage with shift 1 on the input side, two
+// independent networks of size N/2 on the even and the odd positions, and a
+// delta-swap stage with shift 1 on the output side.  Inputs 2k, 2k+1 must be
+// split between the halves, and so must the sources of outputs 2k, 2k+1; the
+// classic looping argument 2-colours these constraints cycle by cycle.
+// Sub-network stages with shift d act on global positions with shift 2d.
+// ---------------------------------------------------------------------------
+void route(std::vector<int> const &src_of, // src_of[i]: input that lands at output i
+           int L, std::vector<std::vector<uint64_t>> &stage_masks, int stage_lo,
+           int pos_mul, int pos_add) {
+  int const N = 1 << L;
+  int const depth = 2 * L - 1;
+  if (L == 0) return;
+  if (L == 1) {
+    if (src_of[0] == 1)
+      stage_masks[stage_lo][0] |= uint64_t{1} << pos_add; // swap positions pos_add, pos_add+pos_mul
+    return;
+  }
+  std::vector<int> dst(N);
+  for (int i = 0; i < N; ++i) dst[src_of[i]] = i;
+  std::vector<int> colour(N, -1); // per input element: 0 even half, 1 odd half
+  for (int start = 0; start < N; ++start) {
+    if (colour[start] != -1) continue;
+    int q = start;
+    int c = 0;
+    while (colour[q] == -1) {
+      colour[q] = c;
+      int const partner = q ^ 1; // input-pair partner takes the other half
+      colour[partner] = 1 - c;
+      // the output paired with partner's output must come from half c
+      int const o = dst[partner] ^ 1;
+      q = src_of[o];
+      // q must take colour c (different from partner's 1 - c)
+    }
+  }
+  int const first = stage_lo, last = stage_lo + depth - 1;
+  std::vector<int> sub_src[2];
+  sub_src[0].resize(N / 2);
+  sub_src[1].resize(N / 2);
+  for (int k = 0; k < N / 2; ++k) {
+    // input stage: swap when the even input belongs to the odd half
+    if (colour[2 * k] == 1) stage_masks[first][0] |= uint64_t{1} << (pos_add + pos_mul * 2 * k);
+    // output stage: swap when output 2k is fed by the odd half
+    int const s0 = src_of[2 * k], s1 = src_of[2 * k + 1];
+    if (colour[s0] == 1) stage_masks[last][0] |= uint64_t{1} << (pos_add + pos_mul * 2 * k);
+    int const even_src = colour[s0] == 0 ? s0 : s1;
+    int const odd_src = colour[s0] == 0 ? s1 : s0;
+    sub_src[0][k] = even_src >> 1;
+    sub_src[1][k] = odd_src >> 1;
+  }
+  route(sub_src[0], L - 1, stage_masks, stage_lo + 1, pos_mul * 2, pos_add);
+  route(sub_src[1], L - 1, stage_masks, stage_lo + 1, pos_mul * 2, pos_add + pos_mul);
+}
+
+// Stage masks and shifts for one permutation p of n sites (bit i of g(x)
+// is bit p[i] of x), padded with fixed points to N = 2^L >= n
+// (Benes.hs:308-314).
+void benes_network(Perm const &p, std::vector<uint64_t> &masks, std::vector<uint64_t> &shifts) {
+  int const n = static_cast<int>(p.size());
+  int L = 0;
+  while ((1 << L) < n) ++L;
+  int const N = 1 << L;
+  int const depth = L == 0 ? 0 : 2 * L - 1;
+  std::vector<int> src_of(N);
+  for (int i = 0; i < N; ++i) src_of[i] = i < n ? p[i] : i;
+  std::vector<std::vector<uint64_t>> stage(depth, std::vector<uint64_t>(1, 0));
+  route(src_of, L, stage, 0, 1, 0);
+  masks.resize(depth);
+  shifts.resize(depth);
+  for (int k = 0; k < depth; ++k) {
+    masks[k] = stage[k][0];
+    shifts[k] = static_cast<uint64_t>(k < L ? (1 << k) : (1 << (depth - 1 - k)));
+  }
+}
+
+uint64_t apply_network(std::vector<uint64_t> const &masks, std::vector<uint64_t> const &shifts,
+                       uint64_t x) {
+  for (size_t k = 0; k < masks.size(); ++k) {
+    unsigned const d = static_cast<unsigned>(shifts[k]);
+    uint64_t const y = ((x >> d) ^ x) & masks[k];
+    x = (x ^ y) ^ (y << d);
+  }
+  return x;
+}
+
+uint64_t permute_direct(Perm const &p, uint64_t x) {
+  uint64_t y = 0;
+  for (size_t i = 0; i < p.size(); ++i) y |= ((x >> p[i]) & 1u) << i;
+  return y;
+}
+
+struct PermGroupOwner {
+  std::vector<uint64_t> masks, shifts;
+  std::vector<double> re, im;
+  std::vector<int32_t> perms; // [size][n]
+};
+
+} // namespace
+
+GroupData *group_data_create(ls_hs_permutation_group const *g, int spin_inversion) {
+  if (g == nullptr) {
+    fail("ls_internal_create_halide_kernel_data: group is NULL");
+    return nullptr;
+  }
+  if (spin_inversion < -1 || spin_inversion > 1) {
+    fail("invalid spin_inversion"); // kernels.c:185-187
+    return nullptr;
+  }
+  if (g->number_bits < 0 || g->number_bits > 64) {
+    fail("too many bits");
+    return nullptr;
+  }
+  if (!ensure_device()) return nullptr;
+  auto d = std::make_unique<GroupData>();
+  d->number_bits = g->number_bits;
+  d->size = g->number_masks;
+  d->depth = g->number_shifts;
+  d->spin_inversion = spin_inversion;
+  d->flip_mask = g->number_bits == 0 ? 0 : (~uint64_t{0}) >> (64 - g->number_bits);
+  int const G = d->size, D = d->depth;
+  d->h_masks.resize(static_cast<size_t>(G) * (D > 0 ? D : 1));
+  for (int e = 0; e < G; ++e)
+    for (int k = 0; k < D; ++k)
+      d->h_masks[static_cast<size_t>(e) * D + k] = g->masks[static_cast<size_t>(k) * G + e];
+  d->h_shifts.assign(g->shifts, g->shifts + D);
+  d->h_eig_re.assign(g->eigvals_re, g->eigvals_re + G);
+  d->h_eig_im.assign(g->eigvals_im, g->eigvals_im + G);
+  // canonical shifts 1, 2, ..., 2^(L-1), ..., 2, 1 (D = 2L - 1)
+  d->canonical_shifts = (D % 2 == 1);
+  if (d->canonical_shifts) {
+    int const L = (D + 1) / 2;
+    for (int k = 0; k < D && d->canonical_shifts; ++k) {
+      int const want = k < L ? (1 << k) : (1 << (D - 1 - k));
+      if (d->h_shifts[k] != static_cast<uint64_t>(want)) d->canonical_shifts = false;
+    }
+  }
+  // 32-bit words are used for depth <= 9: masks must then fit in 32 bits.
+  if (d->canonical_shifts && D <= 9) {
+    for (uint64_t m : d->h_masks)
+      if (m >> 32) d->canonical_shifts = false;
+  }
+  if (!d->d_masks.upload(d->h_masks.data(), d->h_masks.size())) return nullptr;
+  std::vector<uint64_t> sh = d->h_shifts;
+  if (sh.empty()) sh.push_back(0);
+  if (!d->d_shifts.upload(sh.data(), sh.size())) return nullptr;
+  std::vector<double> re = d->h_eig_re, im = d->h_eig_im;
+  if (re.empty()) {
+    re.push_back(1.0);
+    im.push_back(0.0);
+  }
+  if (!d->d_eig_re.upload(re.data(), re.size())) return nullptr;
+  if (!d->d_eig_im.upload(im.data(), im.size())) return nullptr;
+  return d.release();
+}
+
+} // namespace lsamd
+
+using namespace lsamd;
+
+extern "C" ls_hs_permutation_group *
+ls_amd_permutation_group_create(int n, int number_generators, int32_t const *permutations,
+                                int32_t const *sectors) {
+  if (n <= 0 || n > 64) {
+    fail("number of sites must be in [1, 64]");
+    return nullptr;
+  }
+  if (number_generators < 0 || (number_generators > 0 && (permutations == nullptr || sectors == nullptr))) {
+    fail("invalid generators");
+    return nullptr;
+  }
+  // Generators (Group.hs:78-85: sector in [0, periodicity)).
+  std::vector<std::pair<Perm, Phase>> gens;
+  for (int k = 0; k < number_generators; ++k) {
+    Perm p(permutations + static_cast<size_t>(k) * n, permutations + static_cast<size_t>(k + 1) * n);
+    std::vector<char> seen(n, 0);
+    for (int v : p) {
+      if (v < 0 || v >= n || seen[v]) {
+        fail("invalid permutation");
+        return nullptr;
+      }
+      seen[v] = 1;
+    }
+    int const per = order_of(p);
+    if (sectors[k] < 0 || sectors[k] >= per) {
+      fail("invalid sector " + std::to_string(sectors[k]) + "; permutation has periodicity " +
+           std::to_string(per));
+      return nullptr;
+    }
+    gens.emplace_back(p, make_phase(sectors[k], per));
+  }
+  // Closure by breadth-first products h <> g (Group.hs:174-183), kept as an
+  // ordered set of (permutation, phase) so that the result is ascending.
+  std::set<std::pair<Perm, Phase>> interior;
+  std::set<std::pair<Perm, Phase>> boundary;
+  if (!gens.empty()) {
+    Perm id(n);
+    std::iota(id.begin(), id.end(), 0);
+    boundary.insert({id, Phase{0, 1}});
+  }
+  while (!boundary.empty()) {
+    interior.insert(boundary.begin(), boundary.end());
+    std::set<std::pair<Perm, Phase>> next;
+    for (auto const &h : boundary)
+      for (auto const &g : gens) {
+        std::pair<Perm, Phase> s{compose(h.first, g.first), add_phase(h.second, g.second)};
+        if (interior.count(s) == 0) next.insert(std::move(s));
+      }
+    boundary = std::move(next);
+    if (interior.size() > (1u << 20)) {
+      fail("group too large");
+      return nullptr;
+    }
+  }
+  // Consistency (Group.hs:124-131): one phase per permutation and integral
+  // sector for every product.
+  std::map<Perm, Phase> phase_of;
+  for (auto const &s : interior) {
+    auto it = phase_of.find(s.first);
+    if (it != phase_of.end()) {
+      fail("incompatible symmetries");
+      return nullptr;
+    }
+    phase_of.emplace(s.first, s.second);
+  }
+  for (auto const &a : interior)
+    for (auto const &b : interior) {
+      Perm const c = compose(a.first, b.first);
+      Phase const ph = add_phase(a.second, b.second);
+      auto it = phase_of.find(c);
+      if (it == phase_of.end() || !(it->second == ph) ||
+          (static_cast<int64_t>(order_of(c)) * ph.num) % ph.den != 0) {
+        fail("incompatible symmetries");
+        return nullptr;
+      }
+    }
+  auto *owner = new PermGroupOwner();
+  int const G = static_cast<int>(interior.size());
+  int L = 0;
+  while ((1 << L) < n) ++L;
+  int const depth = L == 0 ? 0 : 2 * L - 1;
+  owner->masks.assign(static_cast<size_t>(depth) * G, 0);
+  owner->shifts.assign(depth, 0);
+  owner->re.resize(G);
+  owner->im.resize(G);
+  owner->perms.resize(static_cast<size_t>(G) * n);
+  int e = 0;
+  for (auto const &s : interior) {
+    std::vector<uint64_t> m, sh;
+    benes_network(s.first, m, sh);
+    // self-check of the network against the permutation on basis vectors
+    for (int i = 0; i < n; ++i) {
+      uint64_t const x = uint64_t{1} << i;
+      if (apply_network(m, sh, x) != permute_direct(s.first, x)) {
+        delete owner;
+        fail("internal error: Benes network does not realise the permutation");
+        return nullptr;
+      }
+    }
+    for (int k = 0; k < depth; ++k) owner->masks[static_cast<size_t>(k) * G + e] = m[k];
+    owner->shifts = sh;
+    // Group.hs:115-116
+    double const phi = static_cast<double>(s.second.num) / static_cast<double>(s.second.den);
+    double const arg = -(2.0 * M_PI * phi);
+    owner->re[e] = std::cos(arg);
+    owner->im[e] = std::sin(arg);
+    std::copy(s.first.begin(), s.first.end(), owner->perms.begin() + static_cast<size_t>(e) * n);
+    ++e;
+  }
+  auto *g = new ls_hs_permutation_group{};
+  g->refcount = 1;
+  g->number_bits = n;
+  g->number_shifts = depth;
+  g->number_masks = G;
+  g->masks = owner->masks.data();
+  g->shifts = owner->shifts.data();
+  g->eigvals_re = owner->re.data();
+  g->eigvals_im = owner->im.data();
+  g->haskell_payload = owner;
+  return g;
+}
+
+extern "C" void ls_amd_permutation_group_destroy(ls_hs_permutation_group *g) {
+  if (g == nullptr) return;
+  delete static_cast<PermGroupOwner *>(g->haskell_payload);
+  delete g;
+}
+
+extern "C" int ls_amd_permutation_group_elements(ls_hs_permutation_group const *g, int32_t *out) {
+  if (g == nullptr || out == nullptr || g->haskell_payload == nullptr) return -1;
+  auto const *owner = static_cast<PermGroupOwner const *>(g->haskell_payload);
+  std::copy(owner->perms.begin(), owner->perms.end(), out);
+  return 0;
+}

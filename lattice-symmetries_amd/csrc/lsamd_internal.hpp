@@ -1,0 +1,237 @@
+// lsamd_internal.hpp -- shared declarations of the MI355X engine.
+//
+// Host objects (group, index, basis, operator), error plumbing, and the
+// device views passed to kernels by value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lsamd.h"
+
+namespace lsamd {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+
+// Records `msg` as the last error (ls_amd_last_error).  Returns false so
+// callers can `return fail(...)`.  The reference-named entry points pass the
+// message on to ls_hs_error, the settable handler of reference.c:17-36.
+bool fail(std::string const &msg);
+void set_last_error(std::string const &msg);
+
+#define LSAMD_HIP(call)                                                        \
+  do {                                                                         \
+    hipError_t const _e = (call);                                              \
+    if (_e != hipSuccess) {                                                    \
+      return ::lsamd::fail(std::string("HIP error: ") + hipGetErrorString(_e) + \
+                           " at " + __FILE__ + ":" + std::to_string(__LINE__));  \
+    }                                                                          \
+  } while (0)
+
+// Device set-up; returns false when no usable device.
+bool ensure_device();
+hipStream_t stream();
+
+// Kernel timing by family (see ls_amd_last_kernel_ms).
+enum TimerFamily {
+  kTimerCsrF64 = 0,
+  kTimerCsrF32 = 1,
+  kTimerOtfF64 = 2,
+  kTimerEnumerate = 3,
+  kTimerCsrBuild = 4,
+  kTimerCsrF64Block = 5,
+  kTimerCount = 6
+};
+void timer_begin(int family);
+void timer_end(int family);
+
+// RAII device buffer.
+template <typename T> struct DeviceArray {
+  T *ptr = nullptr;
+  size_t count = 0;
+  DeviceArray() = default;
+  DeviceArray(DeviceArray const &) = delete;
+  DeviceArray &operator=(DeviceArray const &) = delete;
+  DeviceArray(DeviceArray &&o) noexcept : ptr(o.ptr), count(o.count) {
+    o.ptr = nullptr;
+    o.count = 0;
+  }
+  DeviceArray &operator=(DeviceArray &&o) noexcept {
+    if (this != &o) {
+      reset();
+      ptr = o.ptr;
+      count = o.count;
+      o.ptr = nullptr;
+      o.count = 0;
+    }
+    return *this;
+  }
+  ~DeviceArray() { reset(); }
+  bool allocate(size_t n) {
+    reset();
+    if (n == 0) return true;
+    if (hipMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T)) != hipSuccess) {
+      ptr = nullptr;
+      return fail("hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    }
+    count = n;
+    return true;
+  }
+  bool upload(T const *host, size_t n) {
+    if (!allocate(n)) return false;
+    if (n == 0) return true;
+    LSAMD_HIP(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream()));
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    return true;
+  }
+  void reset() {
+    if (ptr != nullptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Device views
+// ---------------------------------------------------------------------------
+
+// Permutation group on the device.  masks are element-major:
+// masks[e * depth + k] is stage k of the Benes network of element e.
+struct GroupView {
+  uint64_t const *masks;
+  double const *eig_re;
+  double const *eig_im;
+  uint64_t const *shifts; // used only by the runtime-depth kernels
+  int size;               // number of group elements (permutations)
+  int depth;              // Benes stages
+  uint64_t flip;          // spin-inversion mask 2^n - 1
+};
+
+// Sorted representatives plus a prefix table over their top bits:
+// states with (x >> shift) == b live in reps[offsets[b] .. offsets[b+1]).
+struct IndexView {
+  uint64_t const *reps;
+  uint32_t const *offsets;
+  int64_t count;
+  int shift;
+  int prefix_bits;
+};
+
+// Non-branching terms, one 64-bit word each.
+struct TermsView {
+  double const *v; // interleaved re, im
+  uint64_t const *m;
+  uint64_t const *r;
+  uint64_t const *x;
+  uint64_t const *s;
+  int count;
+};
+
+// ---------------------------------------------------------------------------
+// Host objects
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kGroupMagic = 0x4c534147u; // "LSAG"
+constexpr uint32_t kIndexMagic = 0x4c534149u; // "LSAI"
+
+// Private data behind ls_internal_create_halide_kernel_data.
+struct GroupData {
+  uint32_t magic = kGroupMagic;
+  int number_bits = 0;
+  int size = 0;
+  int depth = 0;
+  int spin_inversion = 0;
+  bool canonical_shifts = true; // shifts are 1,2,..,N/2,..,2,1
+  uint64_t flip_mask = 0;
+  std::vector<uint64_t> h_masks; // element-major
+  std::vector<uint64_t> h_shifts;
+  std::vector<double> h_eig_re, h_eig_im;
+  DeviceArray<uint64_t> d_masks;
+  DeviceArray<uint64_t> d_shifts;
+  DeviceArray<double> d_eig_re, d_eig_im;
+  GroupView view() const {
+    return GroupView{d_masks.ptr, d_eig_re.ptr, d_eig_im.ptr, d_shifts.ptr,
+                     size, depth, flip_mask};
+  }
+};
+
+// Private data behind ls_hs_create_state_index_binary_search_kernel_data.
+struct IndexData {
+  uint32_t magic = kIndexMagic;
+  int64_t number_states = 0;
+  uint64_t const *h_reps = nullptr; // borrowed host list (may be null)
+  int number_bits = 0;
+  int prefix_bits = 0;
+  int shift = 0;
+  DeviceArray<uint64_t> d_reps_owned;
+  uint64_t const *d_reps = nullptr; // owned copy or borrowed device array
+  DeviceArray<uint32_t> d_offsets;
+  IndexView view() const {
+    return IndexView{d_reps, d_offsets.ptr, number_states, shift, prefix_bits};
+  }
+};
+
+GroupData *group_data_create(ls_hs_permutation_group const *g, int spin_inversion);
+// Builds the device prefix table for sorted device states.
+bool index_build(IndexData &idx, uint64_t const *d_reps, int64_t count,
+                 int number_bits, int prefix_bits);
+// Default prefix width for `count` states of `number_bits` bits.
+int default_prefix_bits(int number_bits, int64_t count);
+
+// Device enumeration of representatives (enumerate.hip).
+struct EnumerationSpec {
+  int number_bits = 0;
+  int hamming_weight = -1;     // -1: every integer in [lower, upper]
+  int spin_inversion = 0;
+  GroupData const *group = nullptr; // null: no permutation symmetries
+  // Spinful fermions with per-species counts: lower number_sites bits are
+  // the up-spins (weight number_up), upper bits the down-spins.
+  bool spinful_per_sector = false;
+  int number_sites = 0;
+  int number_up = -1, number_down = -1;
+};
+// Writes the ascending representatives into `out`; returns false on error.
+bool enumerate_states(EnumerationSpec const &spec, uint64_t lower,
+                      uint64_t upper, DeviceArray<uint64_t> &out,
+                      int64_t &count, int64_t &candidates);
+
+// Orbit kernels on device arrays (orbit.hip).
+bool launch_state_info(GroupData const &g, int64_t count, uint64_t const *d_x,
+                       int64_t x_stride, uint64_t *d_reps, int64_t rep_stride,
+                       double *d_chars, double *d_norms);
+bool launch_is_representative(GroupData const &g, int64_t count,
+                              uint64_t const *d_x, int64_t x_stride,
+                              uint8_t *d_flags, double *d_norms);
+bool launch_state_index(IndexView const &idx, int64_t count,
+                        uint64_t const *d_x, int64_t *d_out);
+
+// Binomial coefficients C(n, k) for 0 <= n, k <= 64, C(n, k) = 0 when
+// n <= 0 or k > n (Basis.hs:487-509).
+uint64_t const *binomials_device();
+uint64_t binomial_host(int n, int k);
+uint64_t fixed_hamming_state_to_index(uint64_t x);
+uint64_t fixed_hamming_index_to_state(uint64_t index, int h);
+
+} // namespace lsamd
+
+// Opaque handles of the additive API.
+struct ls_amd_basis {
+  int number_sites = 0;
+  int hamming_weight = -1;
+  int spin_inversion = 0;
+  std::unique_ptr<lsamd::GroupData> group; // null: no permutations
+  lsamd::DeviceArray<uint64_t> d_states;
+  int64_t number_states = -1;
+  int64_t number_candidates = 0;
+  lsamd::IndexData index;
+  lsamd::DeviceArray<double> d_norms; // state_info norm of each representative
+  bool built() const { return number_states >= 0; }
+};

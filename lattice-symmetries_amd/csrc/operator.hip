@@ -1,0 +1,768 @@
+// operator.hip -- term application, fused product, CSR cache and SpMV.
+//
+// The reference applies H in push form: every representative alpha_i emits
+// (beta, c) pairs, c is rescaled by chi_beta n_beta / n_alpha after
+// state_info, and y[index(rep beta)] += Re(c) x_i with relaxed atomics
+// (chapel/src/BatchedOperator.chpl:207-253,
+// chapel/src/DistributedMatrixVector.chpl:91-143).
+//
+// On the GPU the same matrix is applied in pull form: row i gathers
+// conj(C) x_j over the pairs C = chi_beta v (-1)^.. n_beta / n_alpha that
+// alpha_i itself emits.  For a Hermitian operator M[i, j] = conj(M[j, i]),
+// so every row is computed by one lane without atomics, and for real x the
+// reference's real cast gives the same Re(C) in both forms.
+#include <algorithm>
+#include <cstring>
+
+#include "operator_internal.hpp"
+#include "orbit_device.hpp"
+
+namespace lsamd {
+
+// ---------------------------------------------------------------------------
+// Terms
+// ---------------------------------------------------------------------------
+
+bool Terms::load(ls_hs_nonbranching_terms const *t) {
+  count = 0;
+  h_v.clear();
+  h_m.clear();
+  h_r.clear();
+  h_x.clear();
+  h_s.clear();
+  if (t == nullptr || t->number_terms == 0) return true;
+  if (t->number_bits > 64) return fail("bases with more than 64 bits are not yet implemented");
+  count = t->number_terms;
+  h_v.resize(2 * static_cast<size_t>(count));
+  std::memcpy(h_v.data(), t->v, sizeof(double) * 2 * count);
+  h_m.assign(t->m, t->m + count);
+  h_r.assign(t->r, t->r + count);
+  h_x.assign(t->x, t->x + count);
+  h_s.assign(t->s, t->s + count);
+  return d_v.upload(h_v.data(), h_v.size()) && d_m.upload(h_m.data(), count) &&
+         d_r.upload(h_r.data(), count) && d_x.upload(h_x.data(), count) &&
+         d_s.upload(h_s.data(), count);
+}
+
+uint64_t Terms::fingerprint() const {
+  uint64_t h = 1469598103934665603ull ^ static_cast<uint64_t>(count);
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  for (int i = 0; i < count; ++i) {
+    uint64_t vb[2];
+    std::memcpy(vb, &h_v[2 * i], 16);
+    mix(vb[0]);
+    mix(vb[1]);
+    mix(h_m[i]);
+    mix(h_r[i]);
+    mix(h_x[i]);
+    mix(h_s[i]);
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// reference.c:67-134 on device arrays
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void apply_diag_kernel(TermsView t, int64_t count,
+                                                         uint64_t const *__restrict__ alphas,
+                                                         double *__restrict__ ys,
+                                                         double const *__restrict__ xs) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint64_t const a = alphas[i];
+  double acc = 0;
+  for (int k = 0; k < t.count; ++k) {
+    int const delta = (a & t.m[k]) == t.r[k];
+    if (delta) {
+      int const sign = 1 - 2 * (__popcll(a & t.s[k]) % 2);
+      double const factor =
+          xs != nullptr ? static_cast<double>(delta * sign) * xs[i] : static_cast<double>(delta * sign);
+      acc += t.v[2 * k] * factor;
+    }
+  }
+  ys[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void count_applicable_kernel(TermsView t, int64_t count,
+                                                               uint64_t const *__restrict__ alphas,
+                                                               uint32_t *__restrict__ counts) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint64_t const a = alphas[i];
+  uint32_t c = 0;
+  for (int k = 0; k < t.count; ++k) c += (a & t.m[k]) == t.r[k];
+  counts[i] = c;
+}
+
+__global__ __launch_bounds__(256) void apply_off_diag_kernel(TermsView t, int64_t count,
+                                                             uint64_t const *__restrict__ alphas,
+                                                             double const *__restrict__ xs,
+                                                             int64_t const *__restrict__ offsets,
+                                                             uint64_t *__restrict__ betas,
+                                                             double *__restrict__ coeffs) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint64_t const a = alphas[i];
+  int64_t o = offsets[i];
+  for (int k = 0; k < t.count; ++k) {
+    int const delta = (a & t.m[k]) == t.r[k];
+    if (delta) {
+      int const sign = 1 - 2 * (__popcll(a & t.s[k]) % 2);
+      double const factor =
+          xs != nullptr ? static_cast<double>(delta * sign) * xs[i] : static_cast<double>(delta * sign);
+      coeffs[2 * o] = t.v[2 * k] * factor;
+      coeffs[2 * o + 1] = t.v[2 * k + 1] * factor;
+      betas[o] = a ^ t.x[k];
+      ++o;
+    }
+  }
+}
+
+bool apply_diag_device(Terms const &t, int64_t count, uint64_t const *d_alphas, double *d_ys,
+                       double const *d_xs) {
+  if (count <= 0) return true;
+  if (t.count == 0) {
+    LSAMD_HIP(hipMemsetAsync(d_ys, 0, sizeof(double) * count, stream()));
+    return true;
+  }
+  apply_diag_kernel<<<grid_for(count, 256), 256, 0, stream()>>>(t.view(), count, d_alphas, d_ys, d_xs);
+  LSAMD_HIP(hipGetLastError());
+  return true;
+}
+
+bool apply_off_diag_device(Terms const &t, int64_t count, uint64_t const *d_alphas,
+                           double const *d_xs, DeviceArray<uint64_t> &betas,
+                           DeviceArray<double> &coeffs, DeviceArray<int64_t> &offsets,
+                           int64_t &total) {
+  total = 0;
+  if (!offsets.allocate(static_cast<size_t>(count + 1))) return false;
+  if (t.count == 0 || count == 0) {
+    LSAMD_HIP(hipMemsetAsync(offsets.ptr, 0, sizeof(int64_t) * (count + 1), stream()));
+    betas.reset();
+    coeffs.reset();
+    return true;
+  }
+  DeviceArray<uint32_t> counts;
+  if (!counts.allocate(static_cast<size_t>(count))) return false;
+  count_applicable_kernel<<<grid_for(count, 256), 256, 0, stream()>>>(t.view(), count, d_alphas,
+                                                                     counts.ptr);
+  LSAMD_HIP(hipGetLastError());
+  uint64_t tot = 0;
+  if (!exclusive_scan(counts.ptr, count, reinterpret_cast<uint64_t *>(offsets.ptr), tot)) return false;
+  int64_t const tot_i = static_cast<int64_t>(tot);
+  LSAMD_HIP(hipMemcpyAsync(offsets.ptr + count, &tot_i, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+  if (!betas.allocate(static_cast<size_t>(tot > 0 ? tot : 1))) return false;
+  if (!coeffs.allocate(static_cast<size_t>(2 * (tot > 0 ? tot : 1)))) return false;
+  apply_off_diag_kernel<<<grid_for(count, 256), 256, 0, stream()>>>(t.view(), count, d_alphas, d_xs,
+                                                                   offsets.ptr, betas.ptr, coeffs.ptr);
+  LSAMD_HIP(hipGetLastError());
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  total = tot_i;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Fused product
+// ---------------------------------------------------------------------------
+
+struct OpView {
+  TermsView diag;
+  TermsView off;
+  IndexView idx;
+  double const *norms;
+  int64_t n;
+  uint64_t flip;
+  int identity_index;
+};
+
+OpView make_view(OperatorCore const &op) {
+  return OpView{op.diag.view(), op.off.view(), op.index, op.d_norms, op.number_states,
+                op.number_bits >= 64 ? ~uint64_t{0} : ((uint64_t{1} << op.number_bits) - 1),
+                op.identity_index ? 1 : 0};
+}
+
+// Off-diagonal terms staged in LDS: m, r, x, s then v (re, im).
+struct TermsLds {
+  uint64_t *m, *r, *x, *s;
+  double *v;
+};
+
+__device__ __forceinline__ TermsLds stage_terms(TermsView const &t, uint64_t *smem) {
+  int const T = t.count;
+  TermsLds l{smem, smem + T, smem + 2 * T, smem + 3 * T, reinterpret_cast<double *>(smem + 4 * T)};
+  for (int k = threadIdx.x; k < T; k += blockDim.x) {
+    l.m[k] = t.m[k];
+    l.r[k] = t.r[k];
+    l.x[k] = t.x[k];
+    l.s[k] = t.s[k];
+    l.v[2 * k] = t.v[2 * k];
+    l.v[2 * k + 1] = t.v[2 * k + 1];
+  }
+  __syncthreads();
+  return l;
+}
+
+inline size_t terms_lds_bytes(int T) { return static_cast<size_t>(T) * (4 * 8 + 16); }
+
+__device__ __forceinline__ int next_applicable(TermsLds const &l, int T, uint64_t a, int k) {
+  while (k < T && (a & l.m[k]) != l.r[k]) ++k;
+  return k;
+}
+
+// Coefficient C of term k for row alpha (pull form, without x): returns the
+// representative of beta in `beta` and C in (c_re, c_im).
+template <typename W, int D, int SI, int KIND>
+__device__ __forceinline__ void off_diag_entry(GroupView const &g, OpView const &op,
+                                               TermsLds const &l, int k, uint64_t alpha,
+                                               double n_alpha, uint64_t &beta, double &c_re,
+                                               double &c_im) {
+  double const sign = (__popcll(alpha & l.s[k]) & 1) ? -1.0 : 1.0;
+  double re = l.v[2 * k] * sign, im = l.v[2 * k + 1] * sign;
+  beta = alpha ^ l.x[k];
+  if constexpr (KIND == kInversionOnly) {
+    uint64_t const inv = beta ^ op.flip; // BatchedOperator.chpl:192-199
+    if (inv < beta) {
+      beta = inv;
+      re *= static_cast<double>(SI);
+      im *= static_cast<double>(SI);
+    }
+  } else if constexpr (KIND == kPermutations) {
+    StateInfo const s = state_info_one<W, D, SI>(g, beta); // BatchedOperator.chpl:232-247
+    double const t_re = (re * s.norm) / n_alpha, t_im = (im * s.norm) / n_alpha;
+    re = s.c_re * t_re - s.c_im * t_im;
+    im = s.c_re * t_im + s.c_im * t_re;
+    beta = s.rep;
+  }
+  c_re = re;
+  c_im = im;
+}
+
+__device__ __forceinline__ void record_error(int *err, uint64_t *err_state, uint64_t beta) {
+  if (atomicCAS(err, 0, 1) == 0) *err_state = beta;
+}
+
+template <typename W, int D, int SI, int KIND, bool CPLX>
+__global__ __launch_bounds__(256) void otf_kernel(GroupView g, OpView op,
+                                                  double const *__restrict__ x,
+                                                  double *__restrict__ y, int accumulate,
+                                                  int *err, uint64_t *err_state) {
+  extern __shared__ uint64_t smem[];
+  TermsLds const l = stage_terms(op.off, smem);
+  int const T = op.off.count;
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool const valid = i < op.n;
+  uint64_t const alpha = valid ? op.idx.reps[i] : 0;
+  double base_re = 0, base_im = 0;
+  if (valid) {
+    double const xr = CPLX ? x[2 * i] : x[i];
+    double const xi = CPLX ? x[2 * i + 1] : 0.0;
+    if (op.diag.count > 0) {
+      // reference.c:67-95 term order; Re(v) only
+      for (int k = 0; k < op.diag.count; ++k) {
+        if ((alpha & op.diag.m[k]) != op.diag.r[k]) continue;
+        double const sign = (__popcll(alpha & op.diag.s[k]) & 1) ? -1.0 : 1.0;
+        base_re += op.diag.v[2 * k] * (sign * xr);
+        if constexpr (CPLX) base_im += op.diag.v[2 * k] * (sign * xi);
+      }
+    } else if (accumulate) {
+      base_re = CPLX ? y[2 * i] : y[i];
+      base_im = CPLX ? y[2 * i + 1] : 0.0;
+    }
+  }
+  double const n_alpha = (KIND == kPermutations && valid) ? op.norms[i] : 1.0;
+  double acc_re = 0, acc_im = 0;
+  int k = valid ? next_applicable(l, T, alpha, 0) : T;
+  while (__any(k < T)) {
+    if (k < T) {
+      uint64_t beta;
+      double c_re, c_im;
+      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, beta, c_re, c_im);
+      bool const nz = CPLX ? (c_re != 0 || c_im != 0) : (c_re != 0);
+      if (nz) {
+        int64_t const j = op.identity_index ? static_cast<int64_t>(beta) : index_lookup(op.idx, beta);
+        if (j < 0 || j >= op.n) {
+          record_error(err, err_state, beta);
+        } else if constexpr (CPLX) {
+          double const xr = x[2 * j], xi = x[2 * j + 1];
+          // conj(C) * x_j
+          acc_re += c_re * xr + c_im * xi;
+          acc_im += c_re * xi - c_im * xr;
+        } else {
+          acc_re += c_re * x[j];
+        }
+      }
+      k = next_applicable(l, T, alpha, k + 1);
+    }
+  }
+  if (valid) {
+    if constexpr (CPLX) {
+      y[2 * i] = base_re + acc_re;
+      y[2 * i + 1] = base_im + acc_im;
+    } else {
+      y[i] = base_re + acc_re;
+    }
+  }
+}
+
+// Dispatch over (W, D, SI, KIND): permutation kernels follow the group,
+// the others run on 64-bit words with the spin inversion as SI.
+template <typename F> bool dispatch_op(OperatorCore const &op, F &&f) {
+  if (op.kind == kPermutations) {
+    return dispatch_group(*op.group, [&]<typename W, int D, int SI>() -> bool {
+      return f.template operator()<W, D, SI, kPermutations>();
+    });
+  }
+  if (op.kind == kInversionOnly) {
+    if (op.spin_inversion == 1) return f.template operator()<uint64_t, 0, 1, kInversionOnly>();
+    return f.template operator()<uint64_t, 0, -1, kInversionOnly>();
+  }
+  return f.template operator()<uint64_t, 0, 0, kNoProjection>();
+}
+
+GroupView group_view_or_empty(OperatorCore const &op) {
+  if (op.group != nullptr) return op.group->view();
+  return GroupView{nullptr, nullptr, nullptr, nullptr, 0, 0, 0};
+}
+
+bool check_error(OperatorCore &op) {
+  int err = 0;
+  uint64_t state = 0;
+  LSAMD_HIP(hipMemcpyAsync(&err, op.d_err.ptr, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  LSAMD_HIP(hipMemcpyAsync(&state, op.d_err_state.ptr, sizeof(uint64_t), hipMemcpyDeviceToHost, stream()));
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  if (err != 0) {
+    LSAMD_HIP(hipMemsetAsync(op.d_err.ptr, 0, sizeof(int), stream()));
+    // DistributedMatrixVector.chpl:133-135
+    return fail("invalid index: -1 for state " + std::to_string(state) +
+                " (the operator does not respect the basis symmetries)");
+  }
+  return true;
+}
+
+bool ensure_err(OperatorCore &op) {
+  if (op.d_err.ptr == nullptr) {
+    if (!op.d_err.allocate(1) || !op.d_err_state.allocate(1)) return false;
+    LSAMD_HIP(hipMemsetAsync(op.d_err.ptr, 0, sizeof(int), stream()));
+  }
+  return true;
+}
+
+// state_info norm of every representative (n_alpha of BatchedOperator.chpl:244).
+bool operator_prepare_norms(OperatorCore &op) {
+  if (op.kind != kPermutations || op.d_norms != nullptr) return true;
+  int64_t const n = op.number_states;
+  DeviceArray<uint64_t> reps_out;
+  DeviceArray<double> chars;
+  if (!op.owned_norms.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
+  if (!reps_out.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
+  if (!chars.allocate(static_cast<size_t>(2 * (n > 0 ? n : 1)))) return false;
+  if (!launch_state_info(*op.group, n, op.index.reps, 1, reps_out.ptr, 1, chars.ptr, op.owned_norms.ptr))
+    return false;
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  op.d_norms = op.owned_norms.ptr;
+  return true;
+}
+
+bool operator_matvec_otf(OperatorCore &op, double const *d_x, double *d_y, bool accumulate,
+                         bool complex_values) {
+  if (!ensure_err(op) || !operator_prepare_norms(op)) return false;
+  int64_t const n = op.number_states;
+  if (n == 0) return true;
+  OpView const v = make_view(op);
+  GroupView const g = group_view_or_empty(op);
+  size_t const smem = terms_lds_bytes(op.off.count);
+  timer_begin(kTimerOtfF64);
+  bool ok = dispatch_op(op, [&]<typename W, int D, int SI, int KIND>() -> bool {
+    if (complex_values)
+      otf_kernel<W, D, SI, KIND, true><<<grid_for(n, 256), 256, smem, stream()>>>(
+          g, v, d_x, d_y, accumulate ? 1 : 0, op.d_err.ptr, op.d_err_state.ptr);
+    else
+      otf_kernel<W, D, SI, KIND, false><<<grid_for(n, 256), 256, smem, stream()>>>(
+          g, v, d_x, d_y, accumulate ? 1 : 0, op.d_err.ptr, op.d_err_state.ptr);
+    LSAMD_HIP(hipGetLastError());
+    return true;
+  });
+  timer_end(kTimerOtfF64);
+  if (!ok) return false;
+  return check_error(op);
+}
+
+// ---------------------------------------------------------------------------
+// CSR cache
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kDropped = 0xffffffffu;
+
+template <typename W, int D, int SI, int KIND>
+__global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
+                                                       int64_t const *__restrict__ row_ptr,
+                                                       uint32_t *__restrict__ cols,
+                                                       double *__restrict__ vals,
+                                                       double *__restrict__ diag,
+                                                       uint32_t *__restrict__ kept,
+                                                       int *err, uint64_t *err_state) {
+  extern __shared__ uint64_t smem[];
+  TermsLds const l = stage_terms(op.off, smem);
+  int const T = op.off.count;
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool const valid = i < op.n;
+  uint64_t const alpha = valid ? op.idx.reps[i] : 0;
+  if (valid) {
+    double d = 0;
+    for (int k = 0; k < op.diag.count; ++k) {
+      if ((alpha & op.diag.m[k]) != op.diag.r[k]) continue;
+      double const sign = (__popcll(alpha & op.diag.s[k]) & 1) ? -1.0 : 1.0;
+      d += op.diag.v[2 * k] * sign;
+    }
+    diag[i] = d;
+  }
+  double const n_alpha = (KIND == kPermutations && valid) ? op.norms[i] : 1.0;
+  int64_t o = valid ? row_ptr[i] : 0;
+  uint32_t nkept = 0;
+  int k = valid ? next_applicable(l, T, alpha, 0) : T;
+  while (__any(k < T)) {
+    if (k < T) {
+      uint64_t beta;
+      double c_re, c_im;
+      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, beta, c_re, c_im);
+      uint32_t col = kDropped;
+      double val = 0.0;
+      if (c_re != 0) {
+        int64_t const j = op.identity_index ? static_cast<int64_t>(beta) : index_lookup(op.idx, beta);
+        if (j < 0 || j >= op.n) {
+          record_error(err, err_state, beta);
+        } else {
+          col = static_cast<uint32_t>(j);
+          val = c_re;
+          ++nkept;
+        }
+      }
+      cols[o] = col;
+      vals[o] = val;
+      ++o;
+      k = next_applicable(l, T, alpha, k + 1);
+    }
+  }
+  if (valid) kept[i] = nkept;
+}
+
+__global__ __launch_bounds__(256) void csr_compact_kernel(int64_t n, int64_t const *__restrict__ old_ptr,
+                                                          uint32_t const *__restrict__ old_cols,
+                                                          double const *__restrict__ old_vals,
+                                                          uint64_t const *__restrict__ new_ptr,
+                                                          uint32_t *__restrict__ cols,
+                                                          double *__restrict__ vals) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t o = new_ptr[i];
+  for (int64_t k = old_ptr[i]; k < old_ptr[i + 1]; ++k) {
+    if (old_cols[k] == kDropped) continue;
+    cols[o] = old_cols[k];
+    vals[o] = old_vals[k];
+    ++o;
+  }
+}
+
+__global__ __launch_bounds__(256) void to_f32_kernel(int64_t n, double const *__restrict__ in,
+                                                     float *__restrict__ out) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<float>(in[i]);
+}
+
+bool operator_build_csr(OperatorCore &op, int precision) {
+  if (precision != 32 && precision != 64 && precision != 96) return fail("precision must be 32, 64 or 96");
+  if (op.number_states >= (int64_t{1} << 32) - 1) return fail("CSR columns are 32-bit: too many states");
+  if (!ensure_err(op) || !operator_prepare_norms(op)) return false;
+  int64_t const n = op.number_states;
+  Csr &c = op.csr;
+  c = Csr{};
+  c.rows = n;
+  DeviceArray<uint32_t> counts;
+  DeviceArray<int64_t> row_ptr;
+  if (!counts.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
+  if (!row_ptr.allocate(static_cast<size_t>(n + 1))) return false;
+  if (!c.diag64.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
+  uint64_t upper = 0;
+  if (n > 0) {
+    if (op.off.count > 0) {
+      count_applicable_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(op.off.view(), n, op.index.reps,
+                                                                     counts.ptr);
+      LSAMD_HIP(hipGetLastError());
+    } else {
+      LSAMD_HIP(hipMemsetAsync(counts.ptr, 0, sizeof(uint32_t) * n, stream()));
+    }
+    if (!exclusive_scan(counts.ptr, n, reinterpret_cast<uint64_t *>(row_ptr.ptr), upper)) return false;
+  }
+  int64_t const up = static_cast<int64_t>(upper);
+  LSAMD_HIP(hipMemcpyAsync(row_ptr.ptr + n, &up, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+  DeviceArray<uint32_t> cols;
+  DeviceArray<double> vals;
+  if (!cols.allocate(static_cast<size_t>(up > 0 ? up : 1))) return false;
+  if (!vals.allocate(static_cast<size_t>(up > 0 ? up : 1))) return false;
+  OpView const v = make_view(op);
+  GroupView const g = group_view_or_empty(op);
+  size_t const smem = terms_lds_bytes(op.off.count);
+  timer_begin(kTimerCsrBuild);
+  if (n > 0) {
+    bool ok = dispatch_op(op, [&]<typename W, int D, int SI, int KIND>() -> bool {
+      csr_fill_kernel<W, D, SI, KIND><<<grid_for(n, 256), 256, smem, stream()>>>(
+          g, v, row_ptr.ptr, cols.ptr, vals.ptr, c.diag64.ptr, counts.ptr, op.d_err.ptr,
+          op.d_err_state.ptr);
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    });
+    if (!ok) return false;
+  }
+  timer_end(kTimerCsrBuild);
+  if (!check_error(op)) return false;
+  // Drop zero coefficients (the reference skips c == 0,
+  // DistributedMatrixVector.chpl:127).
+  uint64_t kept = 0;
+  DeviceArray<uint64_t> new_ptr;
+  if (!new_ptr.allocate(static_cast<size_t>(n + 1))) return false;
+  if (n > 0 && !exclusive_scan(counts.ptr, n, new_ptr.ptr, kept)) return false;
+  if (static_cast<int64_t>(kept) != up) {
+    DeviceArray<uint32_t> cols2;
+    DeviceArray<double> vals2;
+    if (!cols2.allocate(static_cast<size_t>(kept > 0 ? kept : 1))) return false;
+    if (!vals2.allocate(static_cast<size_t>(kept > 0 ? kept : 1))) return false;
+    csr_compact_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, vals.ptr,
+                                                               new_ptr.ptr, cols2.ptr, vals2.ptr);
+    LSAMD_HIP(hipGetLastError());
+    int64_t const kk = static_cast<int64_t>(kept);
+    LSAMD_HIP(hipMemcpyAsync(new_ptr.ptr + n, &kk, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    DeviceArray<int64_t> rp2;
+    rp2.ptr = reinterpret_cast<int64_t *>(new_ptr.ptr);
+    rp2.count = new_ptr.count;
+    new_ptr.ptr = nullptr;
+    new_ptr.count = 0;
+    row_ptr = std::move(rp2);
+    cols = std::move(cols2);
+    vals = std::move(vals2);
+  }
+  c.nnz = static_cast<int64_t>(kept);
+  c.row_ptr = std::move(row_ptr);
+  c.cols = std::move(cols);
+  if (precision == 32 || precision == 96) {
+    if (!c.vals32.allocate(static_cast<size_t>(c.nnz > 0 ? c.nnz : 1))) return false;
+    if (!c.diag32.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
+    if (c.nnz > 0) to_f32_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(c.nnz, vals.ptr, c.vals32.ptr);
+    if (n > 0) to_f32_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, c.diag64.ptr, c.diag32.ptr);
+    LSAMD_HIP(hipGetLastError());
+    c.has32 = true;
+  }
+  if (precision == 64 || precision == 96) {
+    c.vals64 = std::move(vals);
+    c.has64 = true;
+  } else {
+    c.diag64.reset();
+  }
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV kernels: LPR lanes share one row
+// ---------------------------------------------------------------------------
+
+template <int LPR>
+__global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                      uint32_t const *__restrict__ cols,
+                                                      double const *__restrict__ vals,
+                                                      double const *__restrict__ diag,
+                                                      double const *__restrict__ x,
+                                                      double *__restrict__ y, int accumulate,
+                                                      int has_diag) {
+  int64_t const gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const row = gid / LPR;
+  int const sub = static_cast<int>(gid % LPR);
+  double acc = 0.0;
+  if (row < n) {
+    int64_t const b = rp[row], e = rp[row + 1];
+    for (int64_t k = b + sub; k < e; k += LPR) acc = fma(vals[k], x[cols[k]], acc);
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  if (row < n && sub == 0) {
+    double const base = has_diag ? diag[row] * x[row] : (accumulate ? y[row] : 0.0);
+    y[row] = base + acc;
+  }
+}
+
+template <int K, int LPR, typename T>
+__global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                        uint32_t const *__restrict__ cols,
+                                                        T const *__restrict__ vals,
+                                                        T const *__restrict__ diag,
+                                                        T const *__restrict__ x,
+                                                        T *__restrict__ y) {
+  int64_t const gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const row = gid / LPR;
+  int const sub = static_cast<int>(gid % LPR);
+  T acc[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) acc[q] = T(0);
+  if (row < n) {
+    int64_t const b = rp[row], e = rp[row + 1];
+    for (int64_t k = b + sub; k < e; k += LPR) {
+      T const v = vals[k];
+      T const *xr = x + static_cast<size_t>(cols[k]) * K;
+#pragma unroll
+      for (int q = 0; q < K; ++q) acc[q] = fma(v, xr[q], acc[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, LPR);
+  if (row < n && sub == 0) {
+    T const d = diag[row];
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+      y[static_cast<size_t>(row) * K + q] = d * x[static_cast<size_t>(row) * K + q] + acc[q];
+  }
+}
+
+template <typename F> bool dispatch_lpr(int lpr, F &&f) {
+  switch (lpr) {
+  case 1: return f.template operator()<1>();
+  case 2: return f.template operator()<2>();
+  case 4: return f.template operator()<4>();
+  case 8: return f.template operator()<8>();
+  case 16: return f.template operator()<16>();
+  case 32: return f.template operator()<32>();
+  case 64: return f.template operator()<64>();
+  default: return fail("lanes per row must be a power of two <= 64");
+  }
+}
+
+template <typename F> bool dispatch_k(int k, F &&f) {
+  switch (k) {
+  case 1: return f.template operator()<1>();
+  case 2: return f.template operator()<2>();
+  case 4: return f.template operator()<4>();
+  case 8: return f.template operator()<8>();
+  case 16: return f.template operator()<16>();
+  default: return fail("block size must be 1, 2, 4, 8 or 16");
+  }
+}
+
+bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool accumulate) {
+  Csr const &c = op.csr;
+  if (!c.has64) return fail("FP64 CSR has not been built");
+  int64_t const n = c.rows;
+  if (n == 0) return true;
+  timer_begin(kTimerCsrF64);
+  bool ok = dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
+    csr_f64_kernel<LPR><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
+        n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, d_x, d_y, accumulate ? 1 : 0,
+        op.diag.count > 0 ? 1 : 0);
+    LSAMD_HIP(hipGetLastError());
+    return true;
+  });
+  timer_end(kTimerCsrF64);
+  return ok;
+}
+
+bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_y) {
+  Csr const &c = op.csr;
+  if (!c.has32) return fail("FP32 CSR has not been built");
+  int64_t const n = c.rows;
+  if (n == 0) return true;
+  timer_begin(kTimerCsrF32);
+  bool ok = dispatch_k(k, [&]<int K>() -> bool {
+    return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
+      csr_block_kernel<K, LPR, float><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
+          n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    });
+  });
+  timer_end(kTimerCsrF32);
+  return ok;
+}
+
+bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *d_y) {
+  Csr const &c = op.csr;
+  if (!c.has64) return fail("FP64 CSR has not been built");
+  int64_t const n = c.rows;
+  if (n == 0) return true;
+  timer_begin(kTimerCsrF64Block);
+  bool ok = dispatch_k(k, [&]<int K>() -> bool {
+    return dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
+      csr_block_kernel<K, LPR, double><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
+          n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, d_x, d_y);
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    });
+  });
+  timer_end(kTimerCsrF64Block);
+  return ok;
+}
+
+// Times every lanes-per-row schedule on the cached matrix and keeps the
+// fastest, separately for the FP64 kernel and the FP32 block-of-8 kernel.
+int operator_autotune(OperatorCore &op) {
+  Csr const &c = op.csr;
+  int64_t const n = c.rows;
+  if (n == 0) return op.lanes_f64;
+  int const cand[] = {2, 4, 8, 16, 32, 64};
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  auto time_it = [&](auto &&launch) -> float {
+    launch();
+    (void)hipEventRecord(e0, stream());
+    for (int r = 0; r < 3; ++r) launch();
+    (void)hipEventRecord(e1, stream());
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return ms;
+  };
+  if (c.has64) {
+    DeviceArray<double> x, y;
+    if (x.allocate(n) && y.allocate(n)) {
+      (void)hipMemsetAsync(x.ptr, 0, sizeof(double) * n, stream());
+      float best = 1e30f;
+      int best_l = op.lanes_f64;
+      for (int l : cand) {
+        op.lanes_f64 = l;
+        float const ms = time_it([&] { (void)operator_csr_f64(op, x.ptr, y.ptr, false); });
+        if (ms < best) {
+          best = ms;
+          best_l = l;
+        }
+      }
+      op.lanes_f64 = best_l;
+    }
+  }
+  if (c.has32) {
+    DeviceArray<float> x, y;
+    if (x.allocate(static_cast<size_t>(n) * 8) && y.allocate(static_cast<size_t>(n) * 8)) {
+      (void)hipMemsetAsync(x.ptr, 0, sizeof(float) * n * 8, stream());
+      float best = 1e30f;
+      int best_l = op.lanes_f32;
+      for (int l : cand) {
+        op.lanes_f32 = l;
+        float const ms = time_it([&] { (void)operator_csr_f32_block(op, 8, x.ptr, y.ptr); });
+        if (ms < best) {
+          best = ms;
+          best_l = l;
+        }
+      }
+      op.lanes_f32 = best_l;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(stream());
+  return op.lanes_f64;
+}
+
+} // namespace lsamd

@@ -1,0 +1,84 @@
+// operator_internal.hpp -- operator objects shared by operator.hip and abi.cpp.
+#pragma once
+
+#include "lsamd_internal.hpp"
+
+namespace lsamd {
+
+// Host + device copy of one set of non-branching terms (single word).
+struct Terms {
+  int count = 0;
+  std::vector<double> h_v; // interleaved re, im
+  std::vector<uint64_t> h_m, h_r, h_x, h_s;
+  DeviceArray<double> d_v;
+  DeviceArray<uint64_t> d_m, d_r, d_x, d_s;
+  bool load(ls_hs_nonbranching_terms const *t);
+  TermsView view() const {
+    return TermsView{d_v.ptr, d_m.ptr, d_r.ptr, d_x.ptr, d_s.ptr, count};
+  }
+  uint64_t fingerprint() const;
+};
+
+// What the product has to do with every beta (BatchedOperator.chpl:264-282).
+enum ProjectionKind { kNoProjection = 0, kInversionOnly = 1, kPermutations = 2 };
+
+struct Csr {
+  int64_t rows = 0;
+  int64_t nnz = 0;
+  DeviceArray<int64_t> row_ptr;
+  DeviceArray<uint32_t> cols;
+  DeviceArray<double> vals64;
+  DeviceArray<float> vals32;
+  DeviceArray<double> diag64;
+  DeviceArray<float> diag32;
+  bool has64 = false, has32 = false;
+};
+
+struct OperatorCore {
+  // basis side
+  int number_bits = 0;
+  int spin_inversion = 0;
+  ProjectionKind kind = kNoProjection;
+  GroupData const *group = nullptr; // kPermutations only
+  IndexView index{};                // representatives + prefix table
+  bool identity_index = false;      // state index is the state itself
+  int64_t number_states = 0;
+  double const *d_norms = nullptr;  // state_info norms of the representatives
+  DeviceArray<double> owned_norms;
+  // terms
+  Terms diag, off;
+  // CSR cache
+  Csr csr;
+  int lanes_f64 = 8;  // lanes per row of the FP64 SpMV
+  int lanes_f32 = 8;  // lanes per row of the FP32 block SpMV
+  DeviceArray<int> d_err; // error flag + offending state
+  DeviceArray<uint64_t> d_err_state;
+};
+
+bool operator_prepare_norms(OperatorCore &op);
+bool operator_matvec_otf(OperatorCore &op, double const *d_x, double *d_y, bool accumulate,
+                         bool complex_values);
+bool operator_build_csr(OperatorCore &op, int precision);
+bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool accumulate);
+bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_y);
+bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *d_y);
+int operator_autotune(OperatorCore &op);
+
+// Term application on device arrays (reference.c:67-134).
+bool apply_diag_device(Terms const &t, int64_t count, uint64_t const *d_alphas, double *d_ys,
+                       double const *d_xs);
+// Two passes: counts -> offsets, then betas / coeffs.  d_offsets has count+1
+// entries (int64).
+bool apply_off_diag_device(Terms const &t, int64_t count, uint64_t const *d_alphas,
+                           double const *d_xs, DeviceArray<uint64_t> &betas,
+                           DeviceArray<double> &coeffs, DeviceArray<int64_t> &offsets,
+                           int64_t &total);
+
+bool exclusive_scan(uint32_t const *d_counts, int64_t n, uint64_t *d_offsets, uint64_t &total);
+
+} // namespace lsamd
+
+struct ls_amd_operator {
+  ls_amd_basis const *basis = nullptr;
+  lsamd::OperatorCore core;
+};

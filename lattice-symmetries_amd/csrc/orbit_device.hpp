@@ -1,0 +1,281 @@
+// orbit_device.hpp -- device functions for group orbits on gfx950.
+//
+// A state is a machine word W (uint32_t when the Benes network spans at most
+// 32 bits, else uint64_t).  One lane owns one state and walks the group
+// elements in their stored (ascending) order; element indices are
+// wave-uniform, so masks and characters are read through scalar loads.
+//
+// The reductions reproduce the order of the reference's Halide kernels
+// (kernels/generator.cpp) literally, so representatives, selected characters
+// and the floating-point norms agree bit for bit with a sequential
+// restatement of those kernels.
+#pragma once
+
+#include "lsamd_internal.hpp"
+
+namespace lsamd {
+
+// Shift of stage k in a Benes network of depth D built for N = 2^L bits:
+// 1, 2, ..., N/2, N/4, ..., 1 (Benes.hs:295-323).
+template <int D> __host__ __device__ constexpr int benes_shift(int k) {
+  return k < (D + 1) / 2 ? (1 << k) : (1 << (D - 1 - k));
+}
+
+// One delta swap (generator.cpp:5-8): exchange bits i and i+d for every i
+// set in m.
+template <typename W>
+__device__ __forceinline__ W delta_swap(W x, W m, int d) {
+  W const y = ((x >> d) ^ x) & m;
+  return (x ^ y) ^ (y << d);
+}
+
+// Apply the Benes network of group element e.  D > 0: compile-time
+// canonical shifts; D == 0: depth and shifts read from the view.
+template <typename W, int D>
+__device__ __forceinline__ W group_apply(GroupView const &g, int e, W x) {
+  if constexpr (D > 0) {
+    uint64_t const *m = g.masks + static_cast<size_t>(e) * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) x = delta_swap<W>(x, static_cast<W>(m[k]), benes_shift<D>(k));
+    return x;
+  } else {
+    int const depth = g.depth;
+    uint64_t const *m = g.masks + static_cast<size_t>(e) * depth;
+    for (int k = 0; k < depth; ++k)
+      x = delta_swap<W>(x, static_cast<W>(m[k]), static_cast<int>(g.shifts[k]));
+    return x;
+  }
+}
+
+// Same, with the masks of element e given as a pointer (e.g. into LDS).
+template <typename W, int D>
+__device__ __forceinline__ W benes_apply_ptr(W const *m, W x,
+                                             uint64_t const *shifts, int depth) {
+  if constexpr (D > 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) x = delta_swap<W>(x, m[k], benes_shift<D>(k));
+    return x;
+  } else {
+    for (int k = 0; k < depth; ++k)
+      x = delta_swap<W>(x, m[k], static_cast<int>(shifts[k]));
+    return x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// state_info (generator.cpp:10-166)
+// ---------------------------------------------------------------------------
+//
+// Eight accumulators (the Halide chunk 2 * natural_vector_size(uint64) of the
+// avx2 target) each see elements e = 8c + j in order; each keeps the smallest
+// image and the first element reaching it, and sums Re(chi) over elements
+// fixing x.  Accumulators are merged left to right (earlier lane wins ties),
+// then the remaining |G| mod 8 elements are folded in one by one.
+// The selected element is kept as an index (2e + flip) and its character is
+// read at the end; sel == -1 means no image is smaller than x (chi = 1).
+
+constexpr int kStateInfoLanes = 8;
+constexpr int kIsRepresentativeLanes = 4;
+
+template <typename W>
+__device__ __forceinline__ void si_update(W &r, int &sel, double &n, W x, W y,
+                                          int code, double re) {
+  if (y < r) {
+    r = y;
+    sel = code;
+  }
+  if (y == x) n = n + re;
+}
+
+struct StateInfo {
+  uint64_t rep;
+  double c_re, c_im;
+  double norm;
+};
+
+template <typename W, int D, int SI>
+__device__ __forceinline__ StateInfo state_info_one(GroupView const &g,
+                                                    uint64_t x64) {
+  constexpr int L = kStateInfoLanes;
+  W const x = static_cast<W>(x64);
+  W const flip = static_cast<W>(g.flip);
+  W r[L];
+  int sel[L];
+  double n[L];
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    r[j] = x;
+    sel[j] = -1;
+    n[j] = 0.0;
+  }
+  int const G = g.size;
+  int const nch = G / L;
+  for (int c = 0; c < nch; ++c) {
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+      int const e = c * L + j;
+      W const y = group_apply<W, D>(g, e, x);
+      double const re = g.eig_re[e];
+      si_update<W>(r[j], sel[j], n[j], x, y, 2 * e, re);
+      if constexpr (SI != 0)
+        si_update<W>(r[j], sel[j], n[j], x, static_cast<W>(y ^ flip), 2 * e + 1,
+                     SI == 1 ? re : -re);
+    }
+  }
+  W rr = r[0];
+  int ss = sel[0];
+  double nn = n[0];
+#pragma unroll
+  for (int j = 1; j < L; ++j) {
+    if (r[j] < rr) {
+      rr = r[j];
+      ss = sel[j];
+    }
+    nn = nn + n[j];
+  }
+  for (int e = nch * L; e < G; ++e) {
+    W const y = group_apply<W, D>(g, e, x);
+    double const re = g.eig_re[e];
+    si_update<W>(rr, ss, nn, x, y, 2 * e, re);
+    if constexpr (SI != 0)
+      si_update<W>(rr, ss, nn, x, static_cast<W>(y ^ flip), 2 * e + 1,
+                   SI == 1 ? re : -re);
+  }
+  StateInfo out;
+  out.rep = static_cast<uint64_t>(rr);
+  if (ss < 0) {
+    out.c_re = 1.0;
+    out.c_im = 0.0;
+  } else {
+    int const e = ss >> 1;
+    double cre = g.eig_re[e], cim = g.eig_im[e];
+    if (SI == -1 && (ss & 1)) {
+      cre = -cre;
+      cim = -cim;
+    }
+    out.c_re = cre;
+    out.c_im = cim;
+  }
+  double const denom = SI == 0 ? static_cast<double>(G) : static_cast<double>(2 * G);
+  out.norm = sqrt(nn / denom);
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// is_representative (generator.cpp:168-272)
+// ---------------------------------------------------------------------------
+//
+// Chunks of four elements (natural_vector_size(uint64) on avx2): within a
+// chunk the terms (y == x) * Re(chi) are summed from 0 in order and the chunk
+// sum is added to the total; the flag is the AND of y >= x (and y^flip >= x).
+// Chunks stop after the first one that clears the flag; the remaining
+// |G| mod 4 elements are always folded in.  is_rep = norm > 0 ? flag : 0.
+
+template <typename W, int D, int SI>
+__device__ __forceinline__ void is_rep_term(GroupView const &g, int e, W x,
+                                            double &t, bool &f) {
+  W const y = group_apply<W, D>(g, e, x);
+  double const re = g.eig_re[e];
+  if constexpr (SI == 0) {
+    t = static_cast<double>(y == x) * re;
+    f = y >= x;
+  } else {
+    W const yf = static_cast<W>(y ^ static_cast<W>(g.flip));
+    double const a = static_cast<double>(y == x);
+    double const b = static_cast<double>(yf == x);
+    t = (SI == 1 ? (a + b) : (a - b)) * re;
+    f = (y >= x) && (yf >= x);
+  }
+}
+
+template <typename W, int D, int SI>
+__device__ __forceinline__ void is_representative_one(GroupView const &g,
+                                                      uint64_t x64,
+                                                      uint8_t &flag_out,
+                                                      double &norm_out) {
+  constexpr int L = kIsRepresentativeLanes;
+  W const x = static_cast<W>(x64);
+  int const G = g.size;
+  int const nch = G / L;
+  double total = 0.0;
+  bool flag = true;
+  for (int c = 0; c < nch; ++c) {
+    if (!flag) break;
+    double lane = 0.0;
+    bool lf = true;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+      double t;
+      bool f;
+      is_rep_term<W, D, SI>(g, c * L + j, x, t, f);
+      lane = lane + t;
+      lf = lf && f;
+    }
+    total = total + lane;
+    flag = flag && lf;
+  }
+  for (int e = nch * L; e < G; ++e) {
+    double t;
+    bool f;
+    is_rep_term<W, D, SI>(g, e, x, t, f);
+    total = total + t;
+    flag = flag && f;
+  }
+  norm_out = total;
+  flag_out = (total > 0) ? static_cast<uint8_t>(flag) : static_cast<uint8_t>(0);
+}
+
+// ---------------------------------------------------------------------------
+// State index lookup over the prefix table (indexing.c:273-325 semantics)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t index_lookup(IndexView const &idx, uint64_t x) {
+  uint64_t const b = x >> idx.shift;
+  if (b >= (uint64_t{1} << idx.prefix_bits)) return -1;
+  int64_t lo = idx.offsets[b];
+  int64_t n = static_cast<int64_t>(idx.offsets[b + 1]) - lo;
+  // branch-free lower bound in [lo, lo + n)
+  while (n > 0) {
+    int64_t const half = n >> 1;
+    uint64_t const v = idx.reps[lo + half];
+    bool const less = v < x;
+    lo = less ? lo + half + 1 : lo;
+    n = less ? n - half - 1 : half;
+  }
+  uint64_t const hi_bucket_end = idx.offsets[b + 1];
+  return (static_cast<uint64_t>(lo) < hi_bucket_end && idx.reps[lo] == x) ? lo : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch on (word type, Benes depth, spin inversion)
+// ---------------------------------------------------------------------------
+//
+// f is a generic callable invoked as f.template operator()<W, D, SI>().
+// Canonical networks of depth 7 and 9 (9..32 sites) run on 32-bit words,
+// depth 11 (33..64 sites) on 64-bit words; anything else uses the runtime
+// depth path.
+template <typename F> bool dispatch_group(GroupData const &g, F &&f) {
+  auto by_si = [&]<typename W, int D>() -> bool {
+    switch (g.spin_inversion) {
+    case 0: return f.template operator()<W, D, 0>();
+    case 1: return f.template operator()<W, D, 1>();
+    case -1: return f.template operator()<W, D, -1>();
+    default: return fail("invalid spin_inversion");
+    }
+  };
+  if (g.canonical_shifts) {
+    switch (g.depth) {
+    case 7: return by_si.template operator()<uint32_t, 7>();
+    case 9: return by_si.template operator()<uint32_t, 9>();
+    case 11: return by_si.template operator()<uint64_t, 11>();
+    default: break;
+    }
+  }
+  return by_si.template operator()<uint64_t, 0>();
+}
+
+inline int grid_for(int64_t n, int block) {
+  int64_t const b = (n + block - 1) / block;
+  return static_cast<int>(b < 1 ? 1 : b);
+}
+
+} // namespace lsamd
