@@ -87,7 +87,7 @@ ENUM_SECTORS = [
     ("chain22_k4", 22, 11, [models.chain_translation(22)], [4], 1),
     ("square4_a2", 16, 8, models.square_generators(4), [0, 0, 1, 1, 0], 0),
     ("chain24_k0", 24, 12, [models.chain_translation(24), models.chain_parity(24)], [0, 0], 1),
-    ("square4_h7", 16, 7, models.square_generators(4), [1, 0, 0, 0, 0], 0),
+    ("square4_h7_pipi", 16, 7, models.square_generators(4), [2, 2, 0, 0, 0], 0),
 ]
 
 
