@@ -281,6 +281,39 @@ int ls_amd_csr_matvec_f64_block(ls_amd_operator *op, int k, double const *d_x,
    on the current matrix; returns the chosen lanes per row. */
 int ls_amd_operator_autotune(ls_amd_operator *op);
 
+/* Eigensolvers (the consumers of the product; the reference drives its
+   product from scipy eigsh, python/lattice_symmetries/__init__.py:591-752,
+   or PRIMME, chapel/src/Diagonalize.chpl:134-162).  FP64 Krylov vectors,
+   orthogonalisation and Ritz solve; the product in the requested form. */
+typedef struct ls_amd_lanczos_result {
+  int steps;              /* Lanczos steps (blocks for block Lanczos) */
+  int refine_steps;       /* FP64 correction steps after the main run */
+  int number_eigenvalues; /* valid entries of eigenvalues */
+  double eigenvalues[8];  /* lowest Ritz values, ascending (the first one
+                             after FP64 correction when requested) */
+  double rayleigh;        /* FP64 Rayleigh quotient of the lowest Ritz vector */
+  double residual_norm;   /* FP64 || H u - rayleigh u || for that vector */
+  double matvec_seconds;  /* wall time spent in products */
+  int64_t matvec_calls;
+  double total_seconds;
+} ls_amd_lanczos_result;
+
+/* Single-vector Lanczos with full re-orthogonalisation.  mode: 0 FP64 CSR,
+   1 FP32 CSR (FP32 vectors in the product only), 2 on-the-fly FP64.
+   Seeded N(0,1) start vector.  Stops when the k lowest Ritz pairs have
+   relative residual bound < tol or after max_steps.  refine_steps > 0 runs
+   that many FP64 Lanczos steps restarted from the lowest Ritz vector.
+   d_ritz_out (device, may be NULL) receives the lowest Ritz vector. */
+int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int k,
+                   double tol, uint64_t seed, int refine_steps,
+                   double *d_ritz_out, ls_amd_lanczos_result *out);
+/* Block Lanczos with block_size (<= 16) right-hand sides per product.
+   mode: 0 FP64 block CSR, 1 FP32 block CSR. */
+int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_size,
+                         int max_blocks, int k, double tol, uint64_t seed,
+                         int refine_steps, double *d_ritz_out,
+                         ls_amd_lanczos_result *out);
+
 /* Timing of the last launched kernel of the given family, in milliseconds
    (HIP events on the engine's stream). family: 0 csr f64, 1 csr f32 block,
    2 on-the-fly f64, 3 enumeration, 4 csr build, 5 csr f64 block. */

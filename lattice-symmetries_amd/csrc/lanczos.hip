@@ -1,0 +1,639 @@
+// lanczos.hip -- Lanczos eigensolvers on the GPU.
+//
+// Consumer of the product (the reference drives it from scipy's eigsh,
+// python/lattice_symmetries/__init__.py:591-752, or PRIMME,
+// chapel/src/Diagonalize.chpl:134-162).  Krylov vectors live in HBM in FP64;
+// orthogonalisation (with full re-orthogonalisation, twice) and all inner
+// products are FP64 with deterministic two-pass reductions.  The product
+// itself is FP64 CSR, FP32 CSR (vectors converted on the fly) or the
+// on-the-fly fused kernel, and the FP32 block kernel for block Lanczos.
+// Ritz values come from a dense symmetric Jacobi solve on the host.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "operator_internal.hpp"
+#include "orbit_device.hpp"
+
+namespace lsamd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kDotBlocks = 1024;
+constexpr int kTile = 8;
+
+// partial[j * kDotBlocks + b] = sum over the rows of block b of V_j . w
+__global__ __launch_bounds__(kBlock) void multi_dot_kernel(double const *__restrict__ V, int64_t ld, int m,
+                                                           double const *__restrict__ w, int64_t n,
+                                                           double *__restrict__ partial) {
+  __shared__ double s[kTile][kBlock / 64];
+  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < m; j0 += kTile) {
+    double acc[kTile];
+#pragma unroll
+    for (int t = 0; t < kTile; ++t) acc[t] = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+      double const wi = w[i];
+#pragma unroll
+      for (int t = 0; t < kTile; ++t)
+        if (j0 + t < m) acc[t] = fma(V[static_cast<int64_t>(j0 + t) * ld + i], wi, acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < kTile; ++t) {
+      double v = acc[t];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) s[t][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kTile && j0 + static_cast<int>(threadIdx.x) < m) {
+      double v = 0.0;
+      for (int q = 0; q < kBlock / 64; ++q) v += s[threadIdx.x][q];
+      partial[static_cast<int64_t>(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// out[j] = sum_b partial[j * nb + b], fixed order
+__global__ __launch_bounds__(kBlock) void reduce_partials_kernel(double const *__restrict__ partial, int nb,
+                                                                 double *__restrict__ out) {
+  __shared__ double s[kBlock];
+  int const j = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) v += partial[static_cast<int64_t>(j) * nb + b];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = s[0];
+}
+
+// w -= sum_j V_j h_j
+__global__ __launch_bounds__(kBlock) void gemv_sub_kernel(double const *__restrict__ V, int64_t ld, int m,
+                                                          double const *__restrict__ h, double *__restrict__ w,
+                                                          int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double acc = w[i];
+  for (int j = 0; j < m; ++j) acc = fma(-V[static_cast<int64_t>(j) * ld + i], h[j], acc);
+  w[i] = acc;
+}
+
+// out = sum_j V_j s_j
+__global__ __launch_bounds__(kBlock) void gemv_kernel(double const *__restrict__ V, int64_t ld, int m,
+                                                      double const *__restrict__ s, double *__restrict__ out,
+                                                      int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int j = 0; j < m; ++j) acc = fma(V[static_cast<int64_t>(j) * ld + i], s[j], acc);
+  out[i] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void scale_kernel(double const *__restrict__ x, double a,
+                                                       double *__restrict__ y, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void axpy_kernel(double a, double const *__restrict__ x,
+                                                      double *__restrict__ y, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) y[i] = fma(a, x[i], y[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void f64_to_f32_kernel(double const *__restrict__ x, float *__restrict__ y,
+                                                            int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) y[i] = static_cast<float>(x[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void f32_to_f64_kernel(float const *__restrict__ x, double *__restrict__ y,
+                                                            int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) y[i] = static_cast<double>(x[i]);
+}
+
+// Seeded N(0, 1) entries: splitmix64 of (seed, i) and Box-Muller.
+__device__ __forceinline__ uint64_t splitmix(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void random_normal_kernel(uint64_t seed, double *__restrict__ x, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint64_t const a = splitmix(seed ^ splitmix(2 * static_cast<uint64_t>(i)));
+  uint64_t const b = splitmix(seed ^ splitmix(2 * static_cast<uint64_t>(i) + 1));
+  double const u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+  double const u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+  x[i] = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+// Block layout conversions: V columns (n x b, column-major) <-> row-major
+// block vectors [n][b] used by the block SpMV.
+__global__ __launch_bounds__(kBlock) void cols_to_rows_f32_kernel(double const *__restrict__ V, int64_t ld, int b,
+                                                                  float *__restrict__ X, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < b; ++q) X[i * b + q] = static_cast<float>(V[static_cast<int64_t>(q) * ld + i]);
+}
+
+__global__ __launch_bounds__(kBlock) void rows_to_cols_f64_kernel(float const *__restrict__ X, int b,
+                                                                  double *__restrict__ W, int64_t ld, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < b; ++q) W[static_cast<int64_t>(q) * ld + i] = static_cast<double>(X[i * b + q]);
+}
+
+__global__ __launch_bounds__(kBlock) void cols_to_rows_f64_kernel(double const *__restrict__ V, int64_t ld, int b,
+                                                                  double *__restrict__ X, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < b; ++q) X[i * b + q] = V[static_cast<int64_t>(q) * ld + i];
+}
+
+__global__ __launch_bounds__(kBlock) void rows_to_cols_f64d_kernel(double const *__restrict__ X, int b,
+                                                                   double *__restrict__ W, int64_t ld, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < b; ++q) W[static_cast<int64_t>(q) * ld + i] = X[i * b + q];
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+
+struct Blas {
+  int64_t n;
+  DeviceArray<double> partial, small;
+  bool init(int64_t n_, int max_m) {
+    n = n_;
+    return partial.allocate(static_cast<size_t>(max_m + kTile) * kDotBlocks) &&
+           small.allocate(static_cast<size_t>(max_m + kTile) + 16 * 16);
+  }
+  int grid() const { return grid_for(n, kBlock); }
+  // h[0..m) = V^T w  (device), copied to host_out when given
+  bool multi_dot(double const *V, int64_t ld, int m, double const *w, double *d_h) {
+    multi_dot_kernel<<<kDotBlocks, kBlock, 0, stream()>>>(V, ld, m, w, n, partial.ptr);
+    reduce_partials_kernel<<<m, kBlock, 0, stream()>>>(partial.ptr, kDotBlocks, d_h);
+    LSAMD_HIP(hipGetLastError());
+    return true;
+  }
+  bool dot(double const *a, double const *b, double &out) {
+    if (!multi_dot(a, n, 1, b, small.ptr)) return false;
+    LSAMD_HIP(hipMemcpyAsync(&out, small.ptr, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    return true;
+  }
+};
+
+// Dense symmetric eigen-decomposition (cyclic Jacobi).  A is m x m row-major;
+// on return w holds the eigenvalues ascending and Z the eigenvectors as
+// columns.
+void jacobi_eigh(std::vector<double> A, int m, std::vector<double> &w, std::vector<double> &Z) {
+  Z.assign(static_cast<size_t>(m) * m, 0.0);
+  for (int i = 0; i < m; ++i) Z[static_cast<size_t>(i) * m + i] = 1.0;
+  auto at = [&](int i, int j) -> double & { return A[static_cast<size_t>(i) * m + j]; };
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) {
+        tot += at(i, j) * at(i, j);
+        if (i != j) off += at(i, j) * at(i, j);
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < m - 1; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        double const apq = at(p, q);
+        if (std::fabs(apq) < 1e-300) continue;
+        double const theta = (at(q, q) - at(p, p)) / (2.0 * apq);
+        double const t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double const c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < m; ++k) {
+          double const akp = at(k, p), akq = at(k, q);
+          at(k, p) = c * akp - s * akq;
+          at(k, q) = s * akp + c * akq;
+        }
+        for (int k = 0; k < m; ++k) {
+          double const apk = at(p, k), aqk = at(q, k);
+          at(p, k) = c * apk - s * aqk;
+          at(q, k) = s * apk + c * aqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          double const zkp = Z[static_cast<size_t>(k) * m + p], zkq = Z[static_cast<size_t>(k) * m + q];
+          Z[static_cast<size_t>(k) * m + p] = c * zkp - s * zkq;
+          Z[static_cast<size_t>(k) * m + q] = s * zkp + c * zkq;
+        }
+      }
+  }
+  std::vector<int> order(m);
+  for (int i = 0; i < m; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return at(a, a) < at(b, b); });
+  w.resize(m);
+  std::vector<double> Zs(static_cast<size_t>(m) * m);
+  for (int c = 0; c < m; ++c) {
+    w[c] = at(order[c], order[c]);
+    for (int r = 0; r < m; ++r) Zs[static_cast<size_t>(r) * m + c] = Z[static_cast<size_t>(r) * m + order[c]];
+  }
+  Z.swap(Zs);
+}
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t) {
+  return std::chrono::duration<double>(Clock::now() - t).count();
+}
+
+// y = H x in the chosen form (FP64 vectors in and out).
+struct Product {
+  OperatorCore *op;
+  int mode; // 0 FP64 CSR, 1 FP32 CSR, 2 on-the-fly FP64
+  DeviceArray<float> xf, yf;
+  double seconds = 0.0;
+  int64_t calls = 0;
+  bool init(OperatorCore &o, int m) {
+    op = &o;
+    mode = m;
+    int64_t const n = o.number_states;
+    if (mode == 1) return xf.allocate(static_cast<size_t>(n)) && yf.allocate(static_cast<size_t>(n));
+    return true;
+  }
+  bool apply(double const *x, double *y) {
+    int64_t const n = op->number_states;
+    auto t0 = Clock::now();
+    bool ok = true;
+    if (mode == 0) {
+      ok = operator_csr_f64(*op, x, y, false);
+    } else if (mode == 1) {
+      f64_to_f32_kernel<<<grid_for(n, kBlock), kBlock, 0, stream()>>>(x, xf.ptr, n);
+      ok = operator_csr_f32_block(*op, 1, xf.ptr, yf.ptr);
+      f32_to_f64_kernel<<<grid_for(n, kBlock), kBlock, 0, stream()>>>(yf.ptr, y, n);
+    } else {
+      ok = operator_matvec_otf(*op, x, y, false, false);
+    }
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    seconds += seconds_since(t0);
+    ++calls;
+    return ok;
+  }
+};
+
+} // namespace
+
+// Single-vector Lanczos.  v0 (device, may be null: seeded random) is the
+// start vector.  Returns the k lowest Ritz values in evals and, when d_ritz
+// is given, the lowest Ritz vector (normalised).
+bool lanczos_run(OperatorCore &op, int mode, int max_steps, int k, double tol, uint64_t seed,
+                 double const *d_v0, double *d_ritz, std::vector<double> &evals, int &steps,
+                 double &matvec_seconds, int64_t &matvec_calls) {
+  int64_t const n = op.number_states;
+  if (n <= 0) return fail("empty basis");
+  int const m_max = static_cast<int>(std::min<int64_t>(max_steps, n));
+  DeviceArray<double> V, w, h;
+  if (!V.allocate(static_cast<size_t>(n) * (m_max + 1)) || !w.allocate(n) || !h.allocate(m_max + 8)) return false;
+  Blas blas;
+  if (!blas.init(n, m_max + 1)) return false;
+  Product prod;
+  if (!prod.init(op, mode)) return false;
+  int const g = grid_for(n, kBlock);
+  if (d_v0 != nullptr) {
+    LSAMD_HIP(hipMemcpyAsync(V.ptr, d_v0, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
+  } else {
+    random_normal_kernel<<<g, kBlock, 0, stream()>>>(seed, V.ptr, n);
+  }
+  double nrm2 = 0;
+  if (!blas.dot(V.ptr, V.ptr, nrm2)) return false;
+  scale_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, 1.0 / std::sqrt(nrm2), V.ptr, n);
+  std::vector<double> alpha, beta;
+  std::vector<double> hh(m_max + 8);
+  std::vector<double> ev, Z;
+  int m = 0;
+  for (int j = 0; j < m_max; ++j) {
+    double *vj = V.ptr + static_cast<int64_t>(j) * n;
+    if (!prod.apply(vj, w.ptr)) return false;
+    double a = 0;
+    if (!blas.dot(vj, w.ptr, a)) return false;
+    alpha.push_back(a);
+    // w -= a v_j + b v_{j-1}, then two passes of full re-orthogonalisation
+    axpy_kernel<<<g, kBlock, 0, stream()>>>(-a, vj, w.ptr, n);
+    if (j > 0) axpy_kernel<<<g, kBlock, 0, stream()>>>(-beta.back(), vj - n, w.ptr, n);
+    for (int pass = 0; pass < 2; ++pass) {
+      if (!blas.multi_dot(V.ptr, n, j + 1, w.ptr, h.ptr)) return false;
+      gemv_sub_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, n, j + 1, h.ptr, w.ptr, n);
+    }
+    double bb = 0;
+    if (!blas.dot(w.ptr, w.ptr, bb)) return false;
+    double const b = std::sqrt(bb);
+    m = j + 1;
+    bool const last = (j + 1 == m_max) || b <= 1e-13 * std::max(1.0, std::fabs(a));
+    bool stop = last;
+    if (!last && (m % 5 == 0)) {
+      std::vector<double> T(static_cast<size_t>(m) * m, 0.0);
+      for (int i = 0; i < m; ++i) {
+        T[static_cast<size_t>(i) * m + i] = alpha[i];
+        if (i + 1 < m) T[static_cast<size_t>(i) * m + i + 1] = T[static_cast<size_t>(i + 1) * m + i] = beta[i];
+      }
+      jacobi_eigh(T, m, ev, Z);
+      int const kk = std::min(k, m);
+      double worst = 0;
+      for (int q = 0; q < kk; ++q)
+        worst = std::max(worst, b * std::fabs(Z[static_cast<size_t>(m - 1) * m + q]) / std::max(1.0, std::fabs(ev[q])));
+      if (m >= k && worst < tol) stop = true;
+    }
+    if (stop) break;
+    beta.push_back(b);
+    scale_kernel<<<g, kBlock, 0, stream()>>>(w.ptr, 1.0 / b, V.ptr + static_cast<int64_t>(j + 1) * n, n);
+  }
+  std::vector<double> T(static_cast<size_t>(m) * m, 0.0);
+  for (int i = 0; i < m; ++i) {
+    T[static_cast<size_t>(i) * m + i] = alpha[i];
+    if (i + 1 < m) T[static_cast<size_t>(i) * m + i + 1] = T[static_cast<size_t>(i + 1) * m + i] = beta[i];
+  }
+  jacobi_eigh(T, m, ev, Z);
+  evals.assign(ev.begin(), ev.begin() + std::min(k, m));
+  steps = m;
+  if (d_ritz != nullptr) {
+    for (int i = 0; i < m; ++i) hh[i] = Z[static_cast<size_t>(i) * m + 0];
+    LSAMD_HIP(hipMemcpyAsync(h.ptr, hh.data(), sizeof(double) * m, hipMemcpyHostToDevice, stream()));
+    gemv_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, n, m, h.ptr, d_ritz, n);
+    double rr = 0;
+    if (!blas.dot(d_ritz, d_ritz, rr)) return false;
+    scale_kernel<<<g, kBlock, 0, stream()>>>(d_ritz, 1.0 / std::sqrt(rr), d_ritz, n);
+  }
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  matvec_seconds = prod.seconds;
+  matvec_calls = prod.calls;
+  return true;
+}
+
+// FP64 Rayleigh quotient and residual norm of a normalised vector.
+bool rayleigh_residual(OperatorCore &op, double const *d_u, double &theta, double &resid) {
+  int64_t const n = op.number_states;
+  DeviceArray<double> w;
+  Blas blas;
+  if (!w.allocate(n) || !blas.init(n, 1)) return false;
+  bool const ok = op.csr.has64 ? operator_csr_f64(op, d_u, w.ptr, false)
+                               : operator_matvec_otf(op, d_u, w.ptr, false, false);
+  if (!ok) return false;
+  double uu = 0;
+  if (!blas.dot(d_u, d_u, uu) || !blas.dot(d_u, w.ptr, theta)) return false;
+  theta /= uu;
+  axpy_kernel<<<grid_for(n, kBlock), kBlock, 0, stream()>>>(-theta, d_u, w.ptr, n);
+  double rr = 0;
+  if (!blas.dot(w.ptr, w.ptr, rr)) return false;
+  resid = std::sqrt(rr / uu);
+  return true;
+}
+
+// Block Lanczos with block size b (<= 16) and the FP32 (mode 1) or FP64
+// (mode 0) block CSR product; FP64 vectors and orthogonalisation.
+bool block_lanczos_run(OperatorCore &op, int mode, int b, int max_blocks, int k, double tol, uint64_t seed,
+                       double *d_ritz, std::vector<double> &evals, int &blocks_done, double &matvec_seconds,
+                       int64_t &matvec_calls) {
+  int64_t const n = op.number_states;
+  if (b < 1 || b > 16) return fail("block size must be in [1, 16]");
+  int const nb_max = static_cast<int>(std::min<int64_t>(max_blocks, std::max<int64_t>(1, n / b - 1)));
+  int const cols = (nb_max + 1) * b;
+  DeviceArray<double> V, W, Hd;
+  DeviceArray<float> Xf, Yf;
+  DeviceArray<double> Xd, Yd;
+  if (!V.allocate(static_cast<size_t>(n) * cols) || !W.allocate(static_cast<size_t>(n) * b) ||
+      !Hd.allocate(static_cast<size_t>(cols) * b + 256))
+    return false;
+  if (mode == 1) {
+    if (!Xf.allocate(static_cast<size_t>(n) * b) || !Yf.allocate(static_cast<size_t>(n) * b)) return false;
+  } else {
+    if (!Xd.allocate(static_cast<size_t>(n) * b) || !Yd.allocate(static_cast<size_t>(n) * b)) return false;
+  }
+  Blas blas;
+  if (!blas.init(n, cols + 16)) return false;
+  int const g = grid_for(n, kBlock);
+  double mv_seconds = 0;
+  int64_t mv_calls = 0;
+  // Orthonormalise the b columns at P in place (modified Gram-Schmidt,
+  // FP64); returns the b x b upper-triangular R (row-major) on the host.
+  auto orthonormalise = [&](double *P, std::vector<double> &R) -> bool {
+    R.assign(static_cast<size_t>(b) * b, 0.0);
+    for (int q = 0; q < b; ++q) {
+      double *pq = P + static_cast<int64_t>(q) * n;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int p = 0; p < q; ++p) {
+          double d = 0;
+          if (!blas.dot(P + static_cast<int64_t>(p) * n, pq, d)) return false;
+          R[static_cast<size_t>(p) * b + q] += d;
+          axpy_kernel<<<g, kBlock, 0, stream()>>>(-d, P + static_cast<int64_t>(p) * n, pq, n);
+        }
+      double nn = 0;
+      if (!blas.dot(pq, pq, nn)) return false;
+      double const r = std::sqrt(nn);
+      R[static_cast<size_t>(q) * b + q] = r;
+      if (r > 0) scale_kernel<<<g, kBlock, 0, stream()>>>(pq, 1.0 / r, pq, n);
+    }
+    return true;
+  };
+  for (int q = 0; q < b; ++q)
+    random_normal_kernel<<<g, kBlock, 0, stream()>>>(seed + 7919ull * q, V.ptr + static_cast<int64_t>(q) * n, n);
+  std::vector<double> R;
+  if (!orthonormalise(V.ptr, R)) return false;
+  // Projected matrix T (block tridiagonal), kept dense on the host.
+  int const tdim = cols;
+  std::vector<double> T(static_cast<size_t>(tdim) * tdim, 0.0);
+  std::vector<double> hbuf(static_cast<size_t>(cols) * b);
+  std::vector<double> ev, Z;
+  int nb = 0;
+  for (int j = 0; j < nb_max; ++j) {
+    double *Vj = V.ptr + static_cast<int64_t>(j) * b * n;
+    auto t0 = Clock::now();
+    if (mode == 1) {
+      cols_to_rows_f32_kernel<<<g, kBlock, 0, stream()>>>(Vj, n, b, Xf.ptr, n);
+      if (!operator_csr_f32_block(op, b, Xf.ptr, Yf.ptr)) return false;
+      rows_to_cols_f64_kernel<<<g, kBlock, 0, stream()>>>(Yf.ptr, b, W.ptr, n, n);
+    } else {
+      cols_to_rows_f64_kernel<<<g, kBlock, 0, stream()>>>(Vj, n, b, Xd.ptr, n);
+      if (!operator_csr_f64_block(op, b, Xd.ptr, Yd.ptr)) return false;
+      rows_to_cols_f64d_kernel<<<g, kBlock, 0, stream()>>>(Yd.ptr, b, W.ptr, n, n);
+    }
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    mv_seconds += seconds_since(t0);
+    mv_calls += 1;
+    // Full re-orthogonalisation of every column of W against V[0 .. (j+1) b),
+    // twice; the coefficients of the first pass against block j give A_j.
+    int const mcols = (j + 1) * b;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int q = 0; q < b; ++q) {
+        double *wq = W.ptr + static_cast<int64_t>(q) * n;
+        if (!blas.multi_dot(V.ptr, n, mcols, wq, Hd.ptr)) return false;
+        LSAMD_HIP(hipMemcpyAsync(hbuf.data() + static_cast<size_t>(q) * cols, Hd.ptr, sizeof(double) * mcols,
+                                 hipMemcpyDeviceToHost, stream()));
+        gemv_sub_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, n, mcols, Hd.ptr, wq, n);
+      }
+      LSAMD_HIP(hipStreamSynchronize(stream()));
+      if (pass == 0) {
+        // A_j = V_j^T H V_j (symmetrised), B_{j-1} from the previous block
+        for (int p = 0; p < b; ++p)
+          for (int q = 0; q < b; ++q) {
+            double const apq = hbuf[static_cast<size_t>(q) * cols + j * b + p];
+            double const aqp = hbuf[static_cast<size_t>(p) * cols + j * b + q];
+            T[static_cast<size_t>(j * b + p) * tdim + j * b + q] = 0.5 * (apq + aqp);
+          }
+      }
+    }
+    nb = j + 1;
+    if (!orthonormalise(W.ptr, R)) return false;
+    // T[(j+1)b + p, j b + q] = R[p][q]
+    if (j + 1 < nb_max + 1)
+      for (int p = 0; p < b; ++p)
+        for (int q = 0; q < b; ++q) {
+          double const r = R[static_cast<size_t>(p) * b + q];
+          T[static_cast<size_t>((j + 1) * b + p) * tdim + j * b + q] = r;
+          T[static_cast<size_t>(j * b + q) * tdim + (j + 1) * b + p] = r;
+        }
+    bool stop = (j + 1 == nb_max);
+    if (!stop && (nb % 4 == 0 || nb * b >= 200)) {
+      int const m = nb * b;
+      std::vector<double> Tm(static_cast<size_t>(m) * m);
+      for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c) Tm[static_cast<size_t>(r) * m + c] = T[static_cast<size_t>(r) * tdim + c];
+      jacobi_eigh(Tm, m, ev, Z);
+      double worst = 0;
+      for (int q = 0; q < std::min(k, m); ++q) {
+        // residual bound ||R_j Z_last,q||
+        double rs = 0;
+        for (int p = 0; p < b; ++p) {
+          double acc = 0;
+          for (int c = 0; c < b; ++c)
+            acc += R[static_cast<size_t>(p) * b + c] * Z[static_cast<size_t>((nb - 1) * b + c) * m + q];
+          rs += acc * acc;
+        }
+        worst = std::max(worst, std::sqrt(rs) / std::max(1.0, std::fabs(ev[q])));
+      }
+      if (worst < tol) stop = true;
+    }
+    if (stop) break;
+    LSAMD_HIP(hipMemcpyAsync(V.ptr + static_cast<int64_t>(j + 1) * b * n, W.ptr, sizeof(double) * n * b,
+                             hipMemcpyDeviceToDevice, stream()));
+  }
+  int const m = nb * b;
+  std::vector<double> Tm(static_cast<size_t>(m) * m);
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) Tm[static_cast<size_t>(r) * m + c] = T[static_cast<size_t>(r) * tdim + c];
+  jacobi_eigh(Tm, m, ev, Z);
+  evals.assign(ev.begin(), ev.begin() + std::min(k, m));
+  blocks_done = nb;
+  if (d_ritz != nullptr) {
+    std::vector<double> s(m);
+    for (int i = 0; i < m; ++i) s[i] = Z[static_cast<size_t>(i) * m + 0];
+    LSAMD_HIP(hipMemcpyAsync(Hd.ptr, s.data(), sizeof(double) * m, hipMemcpyHostToDevice, stream()));
+    gemv_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, n, m, Hd.ptr, d_ritz, n);
+    double rr = 0;
+    if (!blas.dot(d_ritz, d_ritz, rr)) return false;
+    scale_kernel<<<g, kBlock, 0, stream()>>>(d_ritz, 1.0 / std::sqrt(rr), d_ritz, n);
+  }
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  matvec_seconds = mv_seconds;
+  matvec_calls = mv_calls;
+  return true;
+}
+
+} // namespace lsamd
+
+using namespace lsamd;
+
+namespace {
+void fill_result(ls_amd_lanczos_result *out, std::vector<double> const &evals, int steps, double mv_s,
+                 int64_t mv_calls, double total_s) {
+  out->steps = steps;
+  out->number_eigenvalues = static_cast<int>(std::min<size_t>(evals.size(), 8));
+  for (int i = 0; i < 8; ++i) out->eigenvalues[i] = i < static_cast<int>(evals.size()) ? evals[i] : 0.0;
+  out->matvec_seconds = mv_s;
+  out->matvec_calls = mv_calls;
+  out->total_seconds = total_s;
+}
+} // namespace
+
+extern "C" int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int k, double tol, uint64_t seed,
+                              int refine_steps, double *d_ritz_out, ls_amd_lanczos_result *out) {
+  if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
+  if (mode < 0 || mode > 2) return fail("mode must be 0 (FP64 CSR), 1 (FP32 CSR) or 2 (on-the-fly)") ? 0 : -1;
+  if ((mode == 0 && !op->core.csr.has64) || (mode == 1 && !op->core.csr.has32))
+    return fail("build the CSR cache in the requested precision first") ? 0 : -1;
+  auto t0 = Clock::now();
+  int64_t const n = op->core.number_states;
+  DeviceArray<double> u;
+  if (!u.allocate(static_cast<size_t>(n))) return -1;
+  std::vector<double> evals;
+  int steps = 0;
+  double mv = 0;
+  int64_t calls = 0;
+  if (!lanczos_run(op->core, mode, max_steps, std::max(1, std::min(k, 8)), tol, seed, nullptr, u.ptr, evals, steps,
+                   mv, calls))
+    return -1;
+  out->refine_steps = 0;
+  if (refine_steps > 0) {
+    // FP64 correction: a short FP64 Krylov run restarted from the Ritz vector
+    std::vector<double> ev2;
+    int st2 = 0;
+    double mv2 = 0;
+    int64_t c2 = 0;
+    int const form = op->core.csr.has64 ? 0 : 2;
+    if (!lanczos_run(op->core, form, refine_steps, 1, 0.0, seed, u.ptr, u.ptr, ev2, st2, mv2, c2)) return -1;
+    evals[0] = ev2[0];
+    mv += mv2;
+    calls += c2;
+    out->refine_steps = st2;
+  }
+  double theta = 0, resid = 0;
+  if (!rayleigh_residual(op->core, u.ptr, theta, resid)) return -1;
+  fill_result(out, evals, steps, mv, calls, seconds_since(t0));
+  out->rayleigh = theta;
+  out->residual_norm = resid;
+  if (d_ritz_out != nullptr)
+    if (hipMemcpy(d_ritz_out, u.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess) return -1;
+  return 0;
+}
+
+extern "C" int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_size, int max_blocks, int k, double tol,
+                                    uint64_t seed, int refine_steps, double *d_ritz_out, ls_amd_lanczos_result *out) {
+  if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
+  if (mode != 0 && mode != 1) return fail("mode must be 0 (FP64 block CSR) or 1 (FP32 block CSR)") ? 0 : -1;
+  if ((mode == 0 && !op->core.csr.has64) || (mode == 1 && !op->core.csr.has32))
+    return fail("build the CSR cache in the requested precision first") ? 0 : -1;
+  auto t0 = Clock::now();
+  int64_t const n = op->core.number_states;
+  DeviceArray<double> u;
+  if (!u.allocate(static_cast<size_t>(n))) return -1;
+  std::vector<double> evals;
+  int blocks = 0;
+  double mv = 0;
+  int64_t calls = 0;
+  if (!block_lanczos_run(op->core, mode, block_size, max_blocks, std::max(1, std::min(k, 8)), tol, seed, u.ptr, evals,
+                         blocks, mv, calls))
+    return -1;
+  out->refine_steps = 0;
+  if (refine_steps > 0) {
+    std::vector<double> ev2;
+    int st2 = 0;
+    double mv2 = 0;
+    int64_t c2 = 0;
+    int const form = op->core.csr.has64 ? 0 : 2;
+    if (!lanczos_run(op->core, form, refine_steps, 1, 0.0, seed, u.ptr, u.ptr, ev2, st2, mv2, c2)) return -1;
+    evals[0] = ev2[0];
+    mv += mv2;
+    calls += c2;
+    out->refine_steps = st2;
+  }
+  double theta = 0, resid = 0;
+  if (!rayleigh_residual(op->core, u.ptr, theta, resid)) return -1;
+  fill_result(out, evals, blocks, mv, calls, seconds_since(t0));
+  out->rayleigh = theta;
+  out->residual_norm = resid;
+  if (d_ritz_out != nullptr)
+    if (hipMemcpy(d_ritz_out, u.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess) return -1;
+  return 0;
+}

@@ -115,6 +115,20 @@ class ls_hs_operator(C.Structure):
     ]
 
 
+class ls_amd_lanczos_result(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int),
+        ("refine_steps", C.c_int),
+        ("number_eigenvalues", C.c_int),
+        ("eigenvalues", C.c_double * 8),
+        ("rayleigh", C.c_double),
+        ("residual_norm", C.c_double),
+        ("matvec_seconds", C.c_double),
+        ("matvec_calls", C.c_int64),
+        ("total_seconds", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _u64p = C.POINTER(C.c_uint64)
 _i64p = C.POINTER(C.c_int64)
@@ -203,6 +217,10 @@ SIGNATURES = {
     "ls_amd_csr_matvec_f64_block": (C.c_int, [_P, C.c_int, _P, _P]),
     "ls_amd_operator_autotune": (C.c_int, [_P]),
     "ls_amd_last_kernel_ms": (C.c_double, [C.c_int]),
+    "ls_amd_lanczos": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, _P,
+                                 C.POINTER(ls_amd_lanczos_result)]),
+    "ls_amd_block_lanczos": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int,
+                                       _P, C.POINTER(ls_amd_lanczos_result)]),
 }
 
 

@@ -851,6 +851,8 @@ ORA_EXPORT int64_t ora_matvec_push_parallel(
     uint64_t const *mo, uint64_t const *ro, uint64_t const *xo,
     uint64_t const *so, double const *x, double *y, int num_threads) {
   int64_t nnz = 0;
+  if (row0 < 0) row0 = 0;
+  if (row1 > num_states) row1 = num_states;
   /* Diagonal pass first, assigning y (DistributedMatrixVector.chpl:34-71). */
 #pragma omp parallel for schedule(static) reduction(+ : nnz) num_threads(num_threads)
   for (int64_t i = row0; i < row1; ++i) {

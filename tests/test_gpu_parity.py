@@ -231,3 +231,34 @@ def test_matvec_rejects_operator_breaking_symmetry(engine):
     op = lsamd.Operator(sb, expr)
     with pytest.raises(lsamd.EngineError, match="invalid index"):
         op.matvec(np.ones(sb.number_states))
+
+
+def test_lanczos_precision_forms(engine):
+    """FP64 Lanczos, FP32-product Lanczos + FP64 correction, and FP32 block
+    Lanczos + FP64 correction against dense diagonalisation of the oracle's
+    matrix (chain of 20, J1-J2, k = 0, P+, Z+)."""
+    say("lanczos")
+    n, h = 20, 10
+    gens = [models.chain_translation(n), models.chain_parity(n)]
+    S = symmetries(gens, [0, 0])
+    expr = models.j1j2_chain(n, 1.0, 0.5)
+    og = O.Group(n, S.elements(), S.characters.real, S.characters.imag, 1)
+    db = lsamd.DeviceBasis(n, h, 1, S)
+    db.build()
+    reps = db.states
+    diag, off = expr.split()
+    M = O.matrix_dense(og, n, 1, reps, O.Terms(diag), O.Terms(off)).real
+    exact = np.linalg.eigvalsh(M)
+    dop = lsamd.DeviceOperator(db, expr)
+    dop.build_csr(96)
+    r64 = dop.lanczos("fp64", max_steps=300, k=1, tol=1e-13)
+    assert abs(r64["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
+    assert r64["residual_norm"] < 1e-6
+    r32 = dop.lanczos("fp32", max_steps=300, k=1, tol=1e-8, refine_steps=30)
+    assert abs(r32["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
+    rb = dop.block_lanczos("fp32", block_size=8, max_blocks=60, k=4, tol=1e-7, refine_steps=30)
+    assert abs(rb["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
+    # the next three Ritz values of the FP32 block run, at FP32 accuracy
+    assert np.allclose(rb["eigenvalues"][1:4], exact[1:4], rtol=1e-5)
+    rotf = dop.lanczos("otf", max_steps=300, k=1, tol=1e-13)
+    assert abs(rotf["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
