@@ -895,4 +895,57 @@ ORA_EXPORT int64_t ora_matvec_push_parallel(
   return nnz;
 }
 
+/*
+ * Selected rows of y = Re(M) x, each computed from the pairs that its own
+ * representative emits: y_i = d_i x_i + sum Re(C) x_idx(rep beta), with C as
+ * in BatchedOperator.chpl:232-247.  Equal to the push form because Re(M) is
+ * symmetric (checked on dense matrices by the tests); used to spot-check
+ * rows of full-size products.  Returns -1 on a state outside the basis.
+ */
+ORA_EXPORT int ora_matvec_rows(ora_group const *g, int lanes, int64_t num_states,
+                               uint64_t const *reps, int64_t nrows,
+                               int64_t const *rows, int Td, double const *vd,
+                               uint64_t const *md, uint64_t const *rd,
+                               uint64_t const *sd, int To, double const *vo,
+                               uint64_t const *mo, uint64_t const *ro,
+                               uint64_t const *xo, uint64_t const *so,
+                               double const *x, double *out) {
+  int status = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t q = 0; q < nrows; ++q) {
+    int64_t const i = rows[q];
+    uint64_t const a = reps[i];
+    double d = 0;
+    for (int t = 0; t < Td; ++t)
+      if ((a & md[t]) == rd[t]) {
+        int const sign = 1 - 2 * (__builtin_popcountll(a & sd[t]) % 2);
+        d += vd[2 * t] * ((double)sign * x[i]);
+      }
+    uint64_t rr;
+    double ch[2], na;
+    state_info_one(g, lanes, a, &rr, ch, &na);
+    double acc = 0;
+    for (int t = 0; t < To; ++t) {
+      if ((a & mo[t]) != ro[t]) continue;
+      int const sign = 1 - 2 * (__builtin_popcountll(a & so[t]) % 2);
+      uint64_t rb;
+      double nb;
+      state_info_one(g, lanes, a ^ xo[t], &rb, ch, &nb);
+      double const t_re = (vo[2 * t] * (double)sign * nb) / na;
+      double const t_im = (vo[2 * t + 1] * (double)sign * nb) / na;
+      double const c = ch[0] * t_re - ch[1] * t_im;
+      if (c == 0) continue;
+      int64_t idx;
+      ora_state_index(num_states, reps, 1, &rb, &idx);
+      if (idx < 0) {
+        status = -1;
+        continue;
+      }
+      acc += c * x[idx];
+    }
+    out[q] = d + acc;
+  }
+  return status;
+}
+
 ORA_EXPORT int ora_num_threads(void) { return omp_get_max_threads(); }

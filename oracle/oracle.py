@@ -365,3 +365,21 @@ def lanczos_ground_state(matvec, n, steps=100, seed=42):
         V.append(w / b)
     T = np.diag(alphas) + np.diag(betas[:len(alphas) - 1], 1) + np.diag(betas[:len(alphas) - 1], -1)
     return float(np.linalg.eigvalsh(T)[0])
+
+
+def matvec_rows(group, reps, diag: Terms, off: Terms, x, rows):
+    """Selected rows of Re(M) x (see ora_matvec_rows)."""
+    L = lib()
+    L.ora_matvec_rows.restype = C.c_int
+    L.ora_matvec_rows.argtypes = [C.c_void_p, C.c_int, C.c_int64, _u64p, C.c_int64, _i64p,
+                                  C.c_int, _f64p, _u64p, _u64p, _u64p,
+                                  C.c_int, _f64p, _u64p, _u64p, _u64p, _u64p, _f64p, _f64p]
+    reps = np.ascontiguousarray(reps, dtype=np.uint64)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(rows.size)
+    rc = L.ora_matvec_rows(group.h, 8, reps.size, _p(reps, _u64p), rows.size, _p(rows, _i64p),
+                           *diag.args(with_x=False), *off.args(), _p(x, _f64p), _p(out, _f64p))
+    if rc != 0:
+        raise ValueError("a state is not in the basis")
+    return out

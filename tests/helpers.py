@@ -142,3 +142,46 @@ def sector_matrix_numpy(n, perms, chars, spin_inversion, Hfull, states, tol=1e-8
         return np.zeros((0, 0))
     P = np.stack(vecs, axis=1)
     return P.conj().T @ Hfull @ P
+
+
+def _cycles(perm):
+    n = len(perm)
+    seen = [False] * n
+    out = []
+    for i in range(n):
+        if not seen[i]:
+            length = 0
+            j = i
+            while not seen[j]:
+                seen[j] = True
+                j = int(perm[j])
+                length += 1
+            out.append(length)
+    return out
+
+
+def _poly_coeff(cycle_lengths, h):
+    """Coefficient of x^h in prod_c (1 + x^len(c)): states of weight h fixed
+    by a permutation with these cycles."""
+    coeffs = [0] * (h + 1)
+    coeffs[0] = 1
+    for L in cycle_lengths:
+        for w in range(h, L - 1, -1):
+            coeffs[w] += coeffs[w - L]
+    return coeffs[h]
+
+
+def burnside_by_cycles(n, h, perms, chars, spin_inversion):
+    """Sector dimension (1/|G|) sum_g conj(chi_g) fix_h(g) from cycle
+    structures (no enumeration): fix(g) counts weight-h states constant on
+    the cycles of g; fix(flip g) requires even cycles, alternating, each
+    contributing half its length (2 choices per cycle)."""
+    total = 0j
+    for p, c in zip(perms, chars):
+        cyc = _cycles(p)
+        total += np.conj(c) * _poly_coeff(cyc, h)
+        if spin_inversion:
+            if all(L % 2 == 0 for L in cyc) and sum(L // 2 for L in cyc) == h:
+                total += spin_inversion * np.conj(c) * (2 ** len(cyc))
+    order = len(perms) * (2 if spin_inversion else 1)
+    return total / order
