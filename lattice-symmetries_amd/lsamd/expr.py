@@ -31,15 +31,23 @@ class Term:
     s: int
 
     def compose(self, b: "Term") -> "Term":
-        """Operator product self * b (b acts first), NonbranchingTerm.hs:43-58."""
+        """Operator product self * b (b acts first), as NonbranchingTerm.hs:43-58.
+
+        b maps alpha to alpha ^ x_b with sign (-1)^(alpha . s_b), then self
+        applies with sign (-1)^((alpha ^ x_b) . s_a); on the masked bits alpha
+        equals the composite r, so the constant part of the sign is
+        popcount(r & (s_a ^ s_b)) + popcount(x_b & s_a) (mod 2).  The
+        reference writes the first term as popcount(r & s_a & s_b), which is
+        the same for the index-ordered products its canonical form builds
+        (r & s_a = 0 there); this form also holds for products such as
+        c+_j c_i with j > i."""
         a = self
         m = a.m | b.m
         r = b.r | (a.r & ~b.m)
         l = a.l | (b.l & ~a.m)
         x = l ^ r
         s = (a.s ^ b.s) & ~m
-        z = (r & b.s) ^ b.x
-        p = _popcount((r & b.s) ^ (z & a.s))
+        p = _popcount(r & (a.s ^ b.s)) + _popcount(b.x & a.s)
         if (a.r ^ b.l) & a.m & b.m:
             v = 0j
         else:

@@ -271,6 +271,10 @@ typedef struct ora_group {
   int depth;
   uint64_t *masks, *shifts;
   int use_benes;
+  /* Fermionic signs: 0 = spins; 1 = spinless fermions on number_bits sites;
+     2 = spinful fermions, sites 0..n/2-1 up and n/2..n-1 down. */
+  int fermion_species;
+  int32_t *perms; /* size x number_bits, kept for the signs */
 } ora_group;
 
 /* kernels.c:96-98 */
@@ -295,6 +299,8 @@ ORA_EXPORT ora_group *ora_group_create(int n, int size, int32_t const *perms,
     memcpy(g->eig_im, eig_im, sizeof(double) * (size_t)size);
   }
   if (perms != NULL && size > 0) {
+    g->perms = malloc(sizeof(int32_t) * (size_t)size * n);
+    memcpy(g->perms, perms, sizeof(int32_t) * (size_t)size * n);
     g->tab = calloc((size_t)size * 8 * 256, sizeof(uint64_t));
     for (int e = 0; e < size; ++e) {
       int32_t const *p = perms + (size_t)e * n;
@@ -336,6 +342,7 @@ ORA_EXPORT void ora_group_destroy(ora_group *g) {
   free(g->tab);
   free(g->masks);
   free(g->shifts);
+  free(g->perms);
   free(g);
 }
 
@@ -358,10 +365,45 @@ ORA_EXPORT void ora_group_images(ora_group const *g, uint64_t x, uint64_t *ys) {
 /* ------------------------------------------------------------------------ */
 
 /* The per-element contribution `temp` (generator.cpp:201-222). */
+/*
+ * Fermionic sign of group element e on state x: with the state written as
+ * the ordered product of creation operators (bit order, the Jordan-Wigner
+ * order of the sign masks s = bit(i) - 1, Generator.hs:240-247), the element
+ * relabels orbital b as the position i with p[i] = b, and the sign is the
+ * parity of the inversions among the relabelled occupied orbitals.  The
+ * reference has no fermionic symmetries (Basis.hs:941-948); this is the
+ * convention of the build, pinned by brute force in the tests.
+ */
+static int fermion_sign(ora_group const *g, int e, uint64_t x) {
+  if (!g->fermion_species) return 1;
+  int const n = g->number_bits;
+  int32_t const *p = g->perms + (size_t)e * n;
+  int inv_of[64];
+  for (int i = 0; i < n; ++i) inv_of[p[i]] = i;
+  int img[64], k = 0;
+  for (int b = 0; b < n; ++b)
+    if ((x >> b) & 1) img[k++] = inv_of[b];
+  int parity = 0;
+  for (int a = 0; a < k; ++a)
+    for (int c = a + 1; c < k; ++c) parity ^= img[a] > img[c];
+  return parity ? -1 : 1;
+}
+
+ORA_EXPORT int ora_group_set_fermionic(ora_group *g, int species) {
+  if (g->perms == NULL || g->spin_inversion != 0) return -1;
+  g->fermion_species = species;
+  return 0;
+}
+
+ORA_EXPORT int ora_fermion_sign(ora_group const *g, int e, uint64_t x) {
+  return fermion_sign(g, e, x);
+}
+
 static inline void is_rep_temp(ora_group const *g, int e, uint64_t x,
                                double *t_n, int *t_f) {
   uint64_t const y = image_of(g, e, x);
-  double const re = g->eig_re[e];
+  double re = g->eig_re[e];
+  if (g->fermion_species && y == x) re = re * (double)fermion_sign(g, e, x);
   if (g->spin_inversion == 0) {
     *t_n = (double)(y == x) * re;
     *t_f = y >= x;
@@ -442,7 +484,12 @@ static inline void si_step_impl(si_acc *a, uint64_t x, uint64_t y, double c_re,
 /* generator.cpp:42-56 */
 static inline void si_step(ora_group const *g, si_acc *a, uint64_t x, int e) {
   uint64_t const y = image_of(g, e, x);
-  double const re = g->eig_re[e], im = g->eig_im[e];
+  double re = g->eig_re[e], im = g->eig_im[e];
+  if (g->fermion_species && (y < a->r || y == x)) {
+    double const sg = (double)fermion_sign(g, e, x);
+    re = re * sg;
+    im = im * sg;
+  }
   si_step_impl(a, x, y, re, im);
   if (g->spin_inversion == 1) si_step_impl(a, x, y ^ g->flip_mask, re, im);
   else if (g->spin_inversion == -1)

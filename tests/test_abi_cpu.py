@@ -152,6 +152,31 @@ def test_heisenberg_bond_closed_form():
         assert abs(e1.terms[k] - e2.terms[k]) < 1e-15
 
 
+def test_term_products_match_matrix_products():
+    """Term.compose (NonbranchingTerm.hs:43-58) against products of the
+    generator matrices, for every ordering of fermionic and spin operators."""
+    n = 4
+    gens = ([fermion_create(i) for i in range(n)] + [fermion_annihilate(i) for i in range(n)])
+    mats = [H.term_matrix(n, sum(g.split(), [])) for g in gens]
+    for ga, ma in zip(gens, mats):
+        for gb, mb in zip(gens, mats):
+            prod = ga * gb
+            got = H.term_matrix(n, sum(prod.split(), []))
+            assert np.allclose(got, ma @ mb), (ga.terms, gb.terms)
+    spins = [sigma_plus(i) for i in range(n)] + [sigma_minus(i) for i in range(n)] + [sigma_z(i) for i in range(n)]
+    smats = [H.term_matrix(n, sum(g.split(), [])) for g in spins]
+    for ga, ma in zip(spins, smats):
+        for gb, mb in zip(spins, smats):
+            assert np.allclose(H.term_matrix(n, sum((ga * gb).split(), [])), ma @ mb)
+
+
+def test_hubbard_hamiltonian_is_hermitian():
+    expr = models.hubbard(4, models.chain_bonds(4), t=1.0, U=4.0)
+    M = H.term_matrix(8, sum(expr.split(), []))
+    assert np.allclose(M, M.conj().T)
+    assert expr.is_hermitian(1e-15)
+
+
 def test_fermion_terms_anticommute():
     n = 4
     I = H.term_matrix(n, [t for t in Expr.identity().split()[0]])
