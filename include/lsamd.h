@@ -215,6 +215,17 @@ typedef struct ls_amd_basis ls_amd_basis;
 ls_amd_basis *ls_amd_basis_create(int number_sites, int hamming_weight,
                                   int spin_inversion,
                                   ls_hs_permutation_group const *group);
+/* Fermion basis with permutation symmetries (no reference counterpart: the
+   reference has no fermionic symmetries, Basis.hs:941-948).  Spinful when
+   number_down >= 0: bits 0..n-1 are the up species (number_up particles)
+   and bits n..2n-1 the down species (Basis.hs:621-628); spinless when
+   number_down < 0 (number_up particles, or -1 for any number).  The group
+   acts on all bits (the same site permutation on both species); group
+   elements carry the sign of the relabelled creation operators in the
+   Jordan-Wigner order of the term sign masks. */
+ls_amd_basis *ls_amd_fermion_basis_create(int number_sites, int number_up,
+                                          int number_down,
+                                          ls_hs_permutation_group const *group);
 void ls_amd_basis_destroy(ls_amd_basis *b);
 /* Basis.hs:734-774 (min/max state estimates). */
 uint64_t ls_amd_basis_min_state(ls_amd_basis const *b);

@@ -549,6 +549,7 @@ extern "C" ls_amd_basis *ls_amd_basis_create(int number_sites, int hamming_weigh
   if (!ensure_device()) return nullptr;
   auto b = std::make_unique<ls_amd_basis>();
   b->number_sites = number_sites;
+  b->number_bits = number_sites;
   b->hamming_weight = hamming_weight;
   b->spin_inversion = spin_inversion;
   if (group != nullptr && group->number_masks > 0) {
@@ -562,16 +563,55 @@ extern "C" ls_amd_basis *ls_amd_basis_create(int number_sites, int hamming_weigh
   return b.release();
 }
 
+extern "C" ls_amd_basis *ls_amd_fermion_basis_create(int number_sites, int number_up, int number_down,
+                                                    ls_hs_permutation_group const *group) {
+  bool const spinful = number_down >= 0;
+  int const nbits = spinful ? 2 * number_sites : number_sites;
+  if (number_sites <= 0 || nbits > 64) {
+    fail("fermion basis needs 1..64 bits");
+    return nullptr;
+  }
+  if (number_up > number_sites || number_down > number_sites || (spinful && number_up < 0)) {
+    fail("invalid particle numbers");
+    return nullptr;
+  }
+  if (!ensure_device()) return nullptr;
+  auto b = std::make_unique<ls_amd_basis>();
+  b->number_sites = number_sites;
+  b->number_bits = nbits;
+  b->fermion_species = spinful ? 2 : 1;
+  b->number_up = number_up;
+  b->number_down = number_down;
+  b->hamming_weight = spinful ? -1 : number_up;
+  b->spin_inversion = 0;
+  if (group != nullptr && group->number_masks > 0) {
+    if (group->number_bits != nbits) {
+      fail("group must act on all " + std::to_string(nbits) + " bits");
+      return nullptr;
+    }
+    b->group.reset(group_data_create(group, 0));
+    if (!b->group || !group_make_fermionic(*b->group, b->fermion_species)) return nullptr;
+  }
+  return b.release();
+}
+
 extern "C" void ls_amd_basis_destroy(ls_amd_basis *b) { delete b; }
 
-// Basis.hs:734-774 (spin bases)
+// Basis.hs:734-774
 extern "C" uint64_t ls_amd_basis_min_state(ls_amd_basis const *b) {
-  if (b->hamming_weight >= 0) return b->hamming_weight == 0 ? 0 : ((uint64_t{1} << b->hamming_weight) - 1);
+  auto ones = [](int h) { return h <= 0 ? uint64_t{0} : (h >= 64 ? ~uint64_t{0} : ((uint64_t{1} << h) - 1)); };
+  if (b->fermion_species == 2) return (ones(b->number_down) << b->number_sites) | ones(b->number_up);
+  if (b->hamming_weight >= 0) return ones(b->hamming_weight);
   return 0;
 }
 
 extern "C" uint64_t ls_amd_basis_max_state(ls_amd_basis const *b) {
-  int const n = b->number_sites, h = b->hamming_weight;
+  if (b->fermion_species == 2) {
+    int const n = b->number_sites;
+    auto top = [n](int h) { return h <= 0 ? uint64_t{0} : (((uint64_t{1} << h) - 1) << (n - h)); };
+    return (top(b->number_down) << n) | top(b->number_up);
+  }
+  int const n = b->number_bits, h = b->hamming_weight;
   if (h >= 0) {
     uint64_t const ones = h == 0 ? 0 : ((h == 64) ? ~uint64_t{0} : ((uint64_t{1} << h) - 1));
     return b->spin_inversion == 0 ? (ones << (n - h)) : (ones << (n - h - 1));
@@ -581,18 +621,23 @@ extern "C" uint64_t ls_amd_basis_max_state(ls_amd_basis const *b) {
 
 static bool basis_finish(ls_amd_basis *b) {
   b->d_norms.reset();
-  int const nbits = b->number_sites;
+  int const nbits = b->number_bits;
   return index_build(b->index, b->d_states.ptr, b->number_states, nbits,
                      default_prefix_bits(nbits, b->number_states));
 }
 
 extern "C" int64_t ls_amd_basis_build(ls_amd_basis *b, uint64_t lower, uint64_t upper) {
   EnumerationSpec spec;
-  spec.number_bits = b->number_sites;
+  spec.number_bits = b->number_bits;
   spec.number_sites = b->number_sites;
   spec.hamming_weight = b->hamming_weight;
   spec.spin_inversion = b->spin_inversion;
   spec.group = b->group.get();
+  if (b->fermion_species == 2) {
+    spec.spinful_per_sector = true;
+    spec.number_up = b->number_up;
+    spec.number_down = b->number_down;
+  }
   int64_t count = 0, candidates = 0;
   if (!enumerate_states(spec, lower, upper, b->d_states, count, candidates)) return -1;
   b->number_states = count;
@@ -647,7 +692,7 @@ extern "C" ls_amd_operator *ls_amd_operator_create(ls_amd_basis const *b, ls_hs_
   auto op = std::make_unique<ls_amd_operator>();
   op->basis = b;
   OperatorCore &c = op->core;
-  c.number_bits = b->number_sites;
+  c.number_bits = b->number_bits;
   c.spin_inversion = b->spin_inversion;
   c.number_states = b->number_states;
   if (b->group) {
@@ -658,7 +703,7 @@ extern "C" ls_amd_operator *ls_amd_operator_create(ls_amd_basis const *b, ls_hs_
   } else {
     c.kind = kNoProjection;
   }
-  c.identity_index = (!b->group && b->spin_inversion == 0 && b->hamming_weight < 0);
+  c.identity_index = (!b->group && b->spin_inversion == 0 && b->hamming_weight < 0 && b->fermion_species != 2);
   c.index = b->index.view();
   if (!c.diag.load(diag) || !c.off.load(off_diag)) return nullptr;
   return op.release();

@@ -26,32 +26,58 @@
 
 namespace lsamd {
 
+constexpr int kAllIntegers = -1; // hamming: every integer in the range
+constexpr int kSpinful = -2;     // hamming: product of two fixed-weight species
+
 struct EnumParams {
   uint64_t rank0;   // rank of the first candidate (or the state itself)
   int64_t total;    // number of candidates
   int64_t per_lane; // candidates per lane, multiple of 32
   int64_t lanes;
-  int hamming; // -1: consecutive integers
+  int hamming; // >= 0 fixed weight, kAllIntegers, kSpinful
   uint64_t const *binom;
+  // kSpinful: x = (xB << sites) | xA with xA of weight up in [minA, maxA]
+  // and xB of weight down; candidate r is (rB, rA) = divmod(r, countA).
+  int sites, up, down;
+  uint64_t rankA0, countA, rankB0, minA, maxA;
 };
 
-__device__ __forceinline__ uint64_t enum_unrank(EnumParams const &p, uint64_t r) {
-  if (p.hamming < 0) return r;
+__device__ __forceinline__ uint64_t unrank_weight(uint64_t const *binom, uint64_t r, int h) {
   uint64_t state = 0;
   int c = 63;
-  for (int i = p.hamming; i > 0; --i) {
-    while (c >= i && p.binom[c * 65 + i] > r) --c;
+  for (int i = h; i > 0; --i) {
+    while (c >= i && binom[c * 65 + i] > r) --c;
     state |= uint64_t{1} << c;
-    r -= p.binom[c * 65 + i];
+    r -= binom[c * 65 + i];
     --c;
   }
   return state;
 }
 
-__device__ __forceinline__ uint64_t enum_next(int hamming, uint64_t v) {
-  if (hamming < 0) return v + 1;
+__device__ __forceinline__ uint64_t gosper_next(uint64_t v) {
   uint64_t const t = v | (v - 1);
   return (t + 1) | (((~t & (t + 1)) - 1) >> (__builtin_ctzll(v) + 1));
+}
+
+__device__ __forceinline__ uint64_t enum_unrank(EnumParams const &p, uint64_t r) {
+  if (p.hamming == kAllIntegers) return r;
+  if (p.hamming == kSpinful) {
+    uint64_t const xa = unrank_weight(p.binom, p.rankA0 + r % p.countA, p.up);
+    uint64_t const xb = unrank_weight(p.binom, p.rankB0 + r / p.countA, p.down);
+    return (xb << p.sites) | xa;
+  }
+  return unrank_weight(p.binom, r, p.hamming);
+}
+
+__device__ __forceinline__ uint64_t enum_next(EnumParams const &p, uint64_t v) {
+  if (p.hamming == kAllIntegers) return v + 1;
+  if (p.hamming == kSpinful) {
+    uint64_t const amask = (uint64_t{1} << p.sites) - 1;
+    uint64_t const xa = v & amask, xb = v >> p.sites;
+    if (xa == p.maxA) return (gosper_next(xb) << p.sites) | p.minA;
+    return (xb << p.sites) | gosper_next(xa);
+  }
+  return gosper_next(v);
 }
 
 template <typename W, int D, int SI, bool LDS>
@@ -95,7 +121,9 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
       }
       bool ok = y >= x;
       double t = static_cast<double>(y == x) * re;
-      if constexpr (SI != 0) {
+      if constexpr (SI == kFermionic) {
+        if (y == x) t = re * fermion_sign(g, e, static_cast<uint64_t>(x));
+      } else if constexpr (SI != 0) {
         W const yf = static_cast<W>(y ^ flip);
         ok = ok && (yf >= x);
         double const a = static_cast<double>(y == x), b = static_cast<double>(yf == x);
@@ -131,7 +159,7 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
           my_bits[(pos - 1) >> 5] = word;
           word = 0;
         }
-        if (remaining > 0) x = static_cast<W>(enum_next(p.hamming, static_cast<uint64_t>(x)));
+        if (remaining > 0) x = static_cast<W>(enum_next(p, static_cast<uint64_t>(x)));
         e = 0;
         total = 0.0;
         chunk = 0.0;
@@ -160,23 +188,8 @@ __global__ __launch_bounds__(256) void enum_write_kernel(EnumParams p,
   for (int64_t k = 0; k < n; ++k) {
     if (my_bits != nullptr && (k & 31) == 0) word = my_bits[k >> 5];
     if ((word >> (k & 31)) & 1u) out[o++] = x;
-    if (k + 1 < n) x = enum_next(p.hamming, x);
+    if (k + 1 < n) x = enum_next(p, x);
   }
-}
-
-// Spinful fermions with fixed particle numbers per species
-// (StatesEnumeration.chpl:290-326): x = (xB << n) | xA, xB outer, xA inner.
-__global__ __launch_bounds__(256) void enum_spinful_kernel(uint64_t const *__restrict__ binom,
-                                                           int n, int up, int down,
-                                                           uint64_t rankA0, uint64_t countA,
-                                                           uint64_t rankB0, uint64_t total,
-                                                           uint64_t *__restrict__ out) {
-  uint64_t const k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= total) return;
-  EnumParams pa{0, 0, 0, 0, up, binom}, pb{0, 0, 0, 0, down, binom};
-  uint64_t const xa = enum_unrank(pa, rankA0 + k % countA);
-  uint64_t const xb = enum_unrank(pb, rankB0 + k / countA);
-  out[k] = (xb << n) | xa;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,7 +310,12 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
   int const n = spec.number_bits;
   uint64_t const mask = n >= 64 ? ~uint64_t{0} : ((uint64_t{1} << n) - 1);
 
+  bool const projected = spec.group != nullptr && spec.group->size > 0;
+  EnumParams p{};
+  p.binom = binom;
+  uint64_t hi = upper;
   if (spec.spinful_per_sector) {
+    // (StatesEnumeration.chpl:290-326): x = (xB << n) | xA, xB outer
     int const ns = spec.number_sites;
     uint64_t const smask = (uint64_t{1} << ns) - 1;
     uint64_t const minA = lower & smask, maxA = upper & smask;
@@ -305,31 +323,29 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
     if (__builtin_popcountll(minA) != spec.number_up || __builtin_popcountll(maxA) != spec.number_up ||
         __builtin_popcountll(minB) != spec.number_down || __builtin_popcountll(maxB) != spec.number_down)
       return fail("invalid lower/upper for a spinful fermion basis");
-    uint64_t const rA0 = fixed_hamming_state_to_index(minA);
-    uint64_t const countA = fixed_hamming_state_to_index(maxA) - rA0 + 1;
-    uint64_t const rB0 = fixed_hamming_state_to_index(minB);
-    uint64_t const countB = fixed_hamming_state_to_index(maxB) - rB0 + 1;
-    uint64_t const total = countA * countB;
-    if (!out.allocate(static_cast<size_t>(total))) return false;
-    enum_spinful_kernel<<<grid_for(static_cast<int64_t>(total), 256), 256, 0, stream()>>>(
-        binom, ns, spec.number_up, spec.number_down, rA0, countA, rB0, total, out.ptr);
-    LSAMD_HIP(hipGetLastError());
-    LSAMD_HIP(hipStreamSynchronize(stream()));
-    count = candidates = static_cast<int64_t>(total);
-    return true;
+    p.hamming = kSpinful;
+    p.sites = ns;
+    p.up = spec.number_up;
+    p.down = spec.number_down;
+    p.minA = minA;
+    p.maxA = maxA;
+    p.rankA0 = fixed_hamming_state_to_index(minA);
+    p.countA = fixed_hamming_state_to_index(maxA) - p.rankA0 + 1;
+    p.rankB0 = fixed_hamming_state_to_index(minB);
+    uint64_t const countB = fixed_hamming_state_to_index(maxB) - p.rankB0 + 1;
+    p.rank0 = 0;
+    p.total = static_cast<int64_t>(p.countA * countB);
+  } else {
+    if (!projected && spec.spin_inversion != 0) hi = std::min(hi, hi ^ mask); // :276-281
+    if (lower > hi) {
+      out.reset();
+      return true;
+    }
+    p.hamming = spec.hamming_weight;
   }
-
-  bool const projected = spec.group != nullptr && spec.group->size > 0;
-  uint64_t hi = upper;
-  if (!projected && spec.spin_inversion != 0) hi = std::min(hi, hi ^ mask); // :276-281
-  if (lower > hi) {
-    out.reset();
-    return true;
-  }
-  EnumParams p{};
-  p.hamming = spec.hamming_weight;
-  p.binom = binom;
-  if (p.hamming >= 0) {
+  if (p.hamming == kSpinful) {
+    // set above
+  } else if (p.hamming >= 0) {
     if (__builtin_popcountll(lower) != p.hamming || __builtin_popcountll(hi) != p.hamming)
       return fail("lower and upper must have the basis Hamming weight");
     p.rank0 = fixed_hamming_state_to_index(lower);

@@ -229,6 +229,30 @@ GroupData *group_data_create(ls_hs_permutation_group const *g, int spin_inversio
   return d.release();
 }
 
+bool group_make_fermionic(GroupData &g, int species) {
+  if (species != 1 && species != 2) return fail("fermion species must be 1 or 2");
+  if (g.spin_inversion != 0) return fail("fermionic bases have no spin inversion");
+  if (species == 2 && g.number_bits % 2 != 0) return fail("spinful fermions need an even number of bits");
+  int const n = g.number_bits, G = g.size, D = g.depth;
+  std::vector<uint8_t> image(static_cast<size_t>(G) * n);
+  for (int e = 0; e < G; ++e) {
+    std::vector<uint64_t> m(g.h_masks.begin() + static_cast<size_t>(e) * D,
+                            g.h_masks.begin() + static_cast<size_t>(e + 1) * D);
+    for (int b = 0; b < n; ++b) {
+      uint64_t const y = apply_network(m, g.h_shifts, uint64_t{1} << b);
+      if (y == 0 || (y & (y - 1)) != 0) return fail("group element is not a permutation of the bits");
+      image[static_cast<size_t>(e) * n + b] = static_cast<uint8_t>(__builtin_ctzll(y));
+    }
+    if (species == 2) // the same site permutation on both species
+      for (int b = 0; b < n / 2; ++b)
+        if (image[static_cast<size_t>(e) * n + b + n / 2] != image[static_cast<size_t>(e) * n + b] + n / 2)
+          return fail("spinful group elements must act identically on both species");
+  }
+  if (!g.d_image.upload(image.data(), image.size())) return false;
+  g.fermion_species = species;
+  return true;
+}
+
 } // namespace lsamd
 
 using namespace lsamd;

@@ -113,6 +113,10 @@ struct GroupView {
   int size;               // number of group elements (permutations)
   int depth;              // Benes stages
   uint64_t flip;          // spin-inversion mask 2^n - 1
+  // Fermionic bases: image[e * nbits + b] is the position that orbital b
+  // takes under element e (null for spins).
+  uint8_t const *image;
+  int nbits;
 };
 
 // Sorted representatives plus a prefix table over their top bits:
@@ -149,6 +153,10 @@ struct GroupData {
   int size = 0;
   int depth = 0;
   int spin_inversion = 0;
+  // 0: spins; 1: spinless fermions; 2: spinful fermions (the site
+  // permutation acts on both species).  Fermionic groups carry the sign of
+  // the relabelled creation operators (no spin inversion).
+  int fermion_species = 0;
   bool canonical_shifts = true; // shifts are 1,2,..,N/2,..,2,1
   uint64_t flip_mask = 0;
   std::vector<uint64_t> h_masks; // element-major
@@ -157,11 +165,16 @@ struct GroupData {
   DeviceArray<uint64_t> d_masks;
   DeviceArray<uint64_t> d_shifts;
   DeviceArray<double> d_eig_re, d_eig_im;
+  DeviceArray<uint8_t> d_image;
   GroupView view() const {
-    return GroupView{d_masks.ptr, d_eig_re.ptr, d_eig_im.ptr, d_shifts.ptr,
-                     size, depth, flip_mask};
+    return GroupView{d_masks.ptr, d_eig_re.ptr, d_eig_im.ptr, d_shifts.ptr, size, depth,
+                     flip_mask, d_image.ptr, number_bits};
   }
 };
+
+// Marks a group as fermionic (species 1 or 2) and uploads the orbital
+// images; spin inversion must be off.
+bool group_make_fermionic(GroupData &g, int species);
 
 // Private data behind ls_hs_create_state_index_binary_search_kernel_data.
 struct IndexData {
@@ -193,7 +206,8 @@ struct EnumerationSpec {
   int spin_inversion = 0;
   GroupData const *group = nullptr; // null: no permutation symmetries
   // Spinful fermions with per-species counts: lower number_sites bits are
-  // the up-spins (weight number_up), upper bits the down-spins.
+  // the up-spins (weight number_up), upper bits the down-spins.  With a
+  // group the product states are filtered through the orbit test.
   bool spinful_per_sector = false;
   int number_sites = 0;
   int number_up = -1, number_down = -1;
@@ -227,6 +241,9 @@ struct ls_amd_basis {
   int number_sites = 0;
   int hamming_weight = -1;
   int spin_inversion = 0;
+  int fermion_species = 0;        // 0 spins, 1 spinless, 2 spinful fermions
+  int number_up = -1, number_down = -1; // spinful per-species counts
+  int number_bits = 0;
   std::unique_ptr<lsamd::GroupData> group; // null: no permutations
   lsamd::DeviceArray<uint64_t> d_states;
   int64_t number_states = -1;

@@ -77,6 +77,29 @@ __device__ __forceinline__ W benes_apply_ptr(W const *m, W x,
 constexpr int kStateInfoLanes = 8;
 constexpr int kIsRepresentativeLanes = 4;
 
+// SI selects the symmetry flavour of the orbit kernels: 0 no spin
+// inversion, +-1 spin inversion with that character, 2 fermionic signs (no
+// inversion).
+constexpr int kFermionic = 2;
+
+// Sign of group element e on state x for fermions: the state is the ordered
+// product of creation operators (bit order, the Jordan-Wigner order of the
+// sign masks); e relabels orbital b as image[b], and the sign is the parity
+// of the inversions among the relabelled occupied orbitals.
+__device__ __forceinline__ double fermion_sign(GroupView const &g, int e, uint64_t x) {
+  uint8_t const *img = g.image + static_cast<size_t>(e) * g.nbits;
+  uint64_t seen = 0;
+  int parity = 0;
+  while (x != 0) {
+    int const b = __builtin_ctzll(x);
+    x &= x - 1;
+    int const i = img[b];
+    parity ^= __popcll(seen >> i) & 1; // earlier orbitals now placed above i
+    seen |= uint64_t{1} << i;
+  }
+  return parity ? -1.0 : 1.0;
+}
+
 template <typename W>
 __device__ __forceinline__ void si_update(W &r, int &sel, double &n, W x, W y,
                                           int code, double re) {
@@ -110,16 +133,22 @@ __device__ __forceinline__ StateInfo state_info_one(GroupView const &g,
   }
   int const G = g.size;
   int const nch = G / L;
+  constexpr bool kFermi = SI == kFermionic;
   for (int c = 0; c < nch; ++c) {
 #pragma unroll
     for (int j = 0; j < L; ++j) {
       int const e = c * L + j;
       W const y = group_apply<W, D>(g, e, x);
-      double const re = g.eig_re[e];
-      si_update<W>(r[j], sel[j], n[j], x, y, 2 * e, re);
-      if constexpr (SI != 0)
-        si_update<W>(r[j], sel[j], n[j], x, static_cast<W>(y ^ flip), 2 * e + 1,
-                     SI == 1 ? re : -re);
+      double re = g.eig_re[e];
+      if constexpr (kFermi) {
+        if (y == x) re = re * fermion_sign(g, e, x64);
+        si_update<W>(r[j], sel[j], n[j], x, y, 2 * e, re);
+      } else {
+        si_update<W>(r[j], sel[j], n[j], x, y, 2 * e, re);
+        if constexpr (SI != 0)
+          si_update<W>(r[j], sel[j], n[j], x, static_cast<W>(y ^ flip), 2 * e + 1,
+                       SI == 1 ? re : -re);
+      }
     }
   }
   W rr = r[0];
@@ -135,11 +164,16 @@ __device__ __forceinline__ StateInfo state_info_one(GroupView const &g,
   }
   for (int e = nch * L; e < G; ++e) {
     W const y = group_apply<W, D>(g, e, x);
-    double const re = g.eig_re[e];
-    si_update<W>(rr, ss, nn, x, y, 2 * e, re);
-    if constexpr (SI != 0)
-      si_update<W>(rr, ss, nn, x, static_cast<W>(y ^ flip), 2 * e + 1,
-                   SI == 1 ? re : -re);
+    double re = g.eig_re[e];
+    if constexpr (kFermi) {
+      if (y == x) re = re * fermion_sign(g, e, x64);
+      si_update<W>(rr, ss, nn, x, y, 2 * e, re);
+    } else {
+      si_update<W>(rr, ss, nn, x, y, 2 * e, re);
+      if constexpr (SI != 0)
+        si_update<W>(rr, ss, nn, x, static_cast<W>(y ^ flip), 2 * e + 1,
+                     SI == 1 ? re : -re);
+    }
   }
   StateInfo out;
   out.rep = static_cast<uint64_t>(rr);
@@ -153,10 +187,15 @@ __device__ __forceinline__ StateInfo state_info_one(GroupView const &g,
       cre = -cre;
       cim = -cim;
     }
+    if constexpr (kFermi) {
+      double const sg = fermion_sign(g, e, x64);
+      cre = cre * sg;
+      cim = cim * sg;
+    }
     out.c_re = cre;
     out.c_im = cim;
   }
-  double const denom = SI == 0 ? static_cast<double>(G) : static_cast<double>(2 * G);
+  double const denom = (SI == 0 || kFermi) ? static_cast<double>(G) : static_cast<double>(2 * G);
   out.norm = sqrt(nn / denom);
   return out;
 }
@@ -175,8 +214,12 @@ template <typename W, int D, int SI>
 __device__ __forceinline__ void is_rep_term(GroupView const &g, int e, W x,
                                             double &t, bool &f) {
   W const y = group_apply<W, D>(g, e, x);
-  double const re = g.eig_re[e];
-  if constexpr (SI == 0) {
+  double re = g.eig_re[e];
+  if constexpr (SI == kFermionic) {
+    if (y == x) re = re * fermion_sign(g, e, static_cast<uint64_t>(x));
+    t = static_cast<double>(y == x) * re;
+    f = y >= x;
+  } else if constexpr (SI == 0) {
     t = static_cast<double>(y == x) * re;
     f = y >= x;
   } else {
@@ -255,6 +298,7 @@ __device__ __forceinline__ int64_t index_lookup(IndexView const &idx, uint64_t x
 // depth path.
 template <typename F> bool dispatch_group(GroupData const &g, F &&f) {
   auto by_si = [&]<typename W, int D>() -> bool {
+    if (g.fermion_species != 0) return f.template operator()<W, D, kFermionic>();
     switch (g.spin_inversion) {
     case 0: return f.template operator()<W, D, 0>();
     case 1: return f.template operator()<W, D, 1>();

@@ -193,6 +193,7 @@ SIGNATURES = {
     "ls_amd_permutation_group_destroy": (None, [C.POINTER(ls_hs_permutation_group)]),
     "ls_amd_permutation_group_elements": (C.c_int, [C.POINTER(ls_hs_permutation_group), _i32p]),
     "ls_amd_basis_create": (_P, [C.c_int, C.c_int, C.c_int, C.POINTER(ls_hs_permutation_group)]),
+    "ls_amd_fermion_basis_create": (_P, [C.c_int, C.c_int, C.c_int, C.POINTER(ls_hs_permutation_group)]),
     "ls_amd_basis_destroy": (None, [_P]),
     "ls_amd_basis_min_state": (C.c_uint64, [_P]),
     "ls_amd_basis_max_state": (C.c_uint64, [_P]),
