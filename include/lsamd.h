@@ -227,6 +227,15 @@ ls_amd_basis *ls_amd_fermion_basis_create(int number_sites, int number_up,
                                           int number_down,
                                           ls_hs_permutation_group const *group);
 void ls_amd_basis_destroy(ls_amd_basis *b);
+/* Opt-in tolerance for numerically zero norms (no reference counterpart).
+   With eps = 0 (the default) the reference's rule holds: a state is a
+   representative when its stabiliser sum of characters is > 0, and the
+   norm is sqrt(sum / |G|) even when the sum is a tiny negative (NaN).  In
+   sectors with characters other than +-1 such sums are +-1e-16 for states
+   whose exact norm is zero; with eps > 0 sums <= eps count as zero, which
+   keeps those states out of the basis and gives their images norm 0.
+   Call before ls_amd_basis_build. */
+int ls_amd_basis_set_norm_tolerance(ls_amd_basis *b, double eps);
 /* Basis.hs:734-774 (min/max state estimates). */
 uint64_t ls_amd_basis_min_state(ls_amd_basis const *b);
 uint64_t ls_amd_basis_max_state(ls_amd_basis const *b);

@@ -597,6 +597,14 @@ extern "C" ls_amd_basis *ls_amd_fermion_basis_create(int number_sites, int numbe
 
 extern "C" void ls_amd_basis_destroy(ls_amd_basis *b) { delete b; }
 
+extern "C" int ls_amd_basis_set_norm_tolerance(ls_amd_basis *b, double eps) {
+  if (!(eps >= 0.0)) return fail("tolerance must be >= 0") ? 0 : -1;
+  if (!b->group) return fail("basis has no permutation symmetries") ? 0 : -1;
+  b->group->norm_eps = eps;
+  b->d_norms.reset();
+  return 0;
+}
+
 // Basis.hs:734-774
 extern "C" uint64_t ls_amd_basis_min_state(ls_amd_basis const *b) {
   auto ones = [](int h) { return h <= 0 ? uint64_t{0} : (h >= 64 ? ~uint64_t{0} : ((uint64_t{1} << h) - 1)); };

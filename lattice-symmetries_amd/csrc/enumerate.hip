@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
         ++e;
         if (e == G) {
           done = true;
-          accept = total > 0;
+          accept = total > g.norm_eps;
         }
       }
       if (done) {

@@ -117,6 +117,9 @@ struct GroupView {
   // takes under element e (null for spins).
   uint8_t const *image;
   int nbits;
+  // Stabiliser sums <= norm_eps count as zero when norm_eps > 0 (0: the
+  // reference's rule, norm > 0 and sqrt of the raw sum).
+  double norm_eps;
 };
 
 // Sorted representatives plus a prefix table over their top bits:
@@ -157,6 +160,7 @@ struct GroupData {
   // permutation acts on both species).  Fermionic groups carry the sign of
   // the relabelled creation operators (no spin inversion).
   int fermion_species = 0;
+  double norm_eps = 0.0;
   bool canonical_shifts = true; // shifts are 1,2,..,N/2,..,2,1
   uint64_t flip_mask = 0;
   std::vector<uint64_t> h_masks; // element-major
@@ -168,7 +172,7 @@ struct GroupData {
   DeviceArray<uint8_t> d_image;
   GroupView view() const {
     return GroupView{d_masks.ptr, d_eig_re.ptr, d_eig_im.ptr, d_shifts.ptr, size, depth,
-                     flip_mask, d_image.ptr, number_bits};
+                     flip_mask, d_image.ptr, number_bits, norm_eps};
   }
 };
 

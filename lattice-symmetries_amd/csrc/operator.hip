@@ -397,7 +397,7 @@ bool operator_matvec_otf(OperatorCore &op, double const *d_x, double *d_y, bool 
 
 constexpr uint32_t kDropped = 0xffffffffu;
 
-template <typename W, int D, int SI, int KIND>
+template <typename W, int D, int SI, int KIND, bool CPLX>
 __global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
                                                        int64_t const *__restrict__ row_ptr,
                                                        uint32_t *__restrict__ cols,
@@ -430,19 +430,25 @@ __global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
       double c_re, c_im;
       off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, beta, c_re, c_im);
       uint32_t col = kDropped;
-      double val = 0.0;
-      if (c_re != 0) {
+      double val_re = 0.0, val_im = 0.0;
+      if (CPLX ? (c_re != 0 || c_im != 0) : (c_re != 0)) {
         int64_t const j = op.identity_index ? static_cast<int64_t>(beta) : index_lookup(op.idx, beta);
         if (j < 0 || j >= op.n) {
           record_error(err, err_state, beta);
         } else {
           col = static_cast<uint32_t>(j);
-          val = c_re;
+          val_re = c_re;
+          val_im = -c_im; // pull form: M[i, j] = conj(C)
           ++nkept;
         }
       }
       cols[o] = col;
-      vals[o] = val;
+      if constexpr (CPLX) {
+        vals[2 * o] = val_re;
+        vals[2 * o + 1] = val_im;
+      } else {
+        vals[o] = val_re;
+      }
       ++o;
       k = next_applicable(l, T, alpha, k + 1);
     }
@@ -455,14 +461,14 @@ __global__ __launch_bounds__(256) void csr_compact_kernel(int64_t n, int64_t con
                                                           double const *__restrict__ old_vals,
                                                           uint64_t const *__restrict__ new_ptr,
                                                           uint32_t *__restrict__ cols,
-                                                          double *__restrict__ vals) {
+                                                          double *__restrict__ vals, int width) {
   int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint64_t o = new_ptr[i];
   for (int64_t k = old_ptr[i]; k < old_ptr[i + 1]; ++k) {
     if (old_cols[k] == kDropped) continue;
     cols[o] = old_cols[k];
-    vals[o] = old_vals[k];
+    for (int q = 0; q < width; ++q) vals[o * width + q] = old_vals[k * width + q];
     ++o;
   }
 }
@@ -474,7 +480,10 @@ __global__ __launch_bounds__(256) void to_f32_kernel(int64_t n, double const *__
 }
 
 bool operator_build_csr(OperatorCore &op, int precision) {
-  if (precision != 32 && precision != 64 && precision != 96) return fail("precision must be 32, 64 or 96");
+  if (precision != 32 && precision != 64 && precision != 96 && precision != 128)
+    return fail("precision must be 32, 64, 96 (real) or 128 (complex)");
+  bool const cplx = precision == 128;
+  int const width = cplx ? 2 : 1;
   if (op.number_states >= (int64_t{1} << 32) - 1) return fail("CSR columns are 32-bit: too many states");
   if (!ensure_err(op) || !operator_prepare_norms(op)) return false;
   int64_t const n = op.number_states;
@@ -502,16 +511,21 @@ bool operator_build_csr(OperatorCore &op, int precision) {
   DeviceArray<uint32_t> cols;
   DeviceArray<double> vals;
   if (!cols.allocate(static_cast<size_t>(up > 0 ? up : 1))) return false;
-  if (!vals.allocate(static_cast<size_t>(up > 0 ? up : 1))) return false;
+  if (!vals.allocate(static_cast<size_t>(width * (up > 0 ? up : 1)))) return false;
   OpView const v = make_view(op);
   GroupView const g = group_view_or_empty(op);
   size_t const smem = terms_lds_bytes(op.off.count);
   timer_begin(kTimerCsrBuild);
   if (n > 0) {
     bool ok = dispatch_op(op, [&]<typename W, int D, int SI, int KIND>() -> bool {
-      csr_fill_kernel<W, D, SI, KIND><<<grid_for(n, 256), 256, smem, stream()>>>(
-          g, v, row_ptr.ptr, cols.ptr, vals.ptr, c.diag64.ptr, counts.ptr, op.d_err.ptr,
-          op.d_err_state.ptr);
+      if (cplx)
+        csr_fill_kernel<W, D, SI, KIND, true><<<grid_for(n, 256), 256, smem, stream()>>>(
+            g, v, row_ptr.ptr, cols.ptr, vals.ptr, c.diag64.ptr, counts.ptr, op.d_err.ptr,
+            op.d_err_state.ptr);
+      else
+        csr_fill_kernel<W, D, SI, KIND, false><<<grid_for(n, 256), 256, smem, stream()>>>(
+            g, v, row_ptr.ptr, cols.ptr, vals.ptr, c.diag64.ptr, counts.ptr, op.d_err.ptr,
+            op.d_err_state.ptr);
       LSAMD_HIP(hipGetLastError());
       return true;
     });
@@ -529,9 +543,9 @@ bool operator_build_csr(OperatorCore &op, int precision) {
     DeviceArray<uint32_t> cols2;
     DeviceArray<double> vals2;
     if (!cols2.allocate(static_cast<size_t>(kept > 0 ? kept : 1))) return false;
-    if (!vals2.allocate(static_cast<size_t>(kept > 0 ? kept : 1))) return false;
+    if (!vals2.allocate(static_cast<size_t>(width * (kept > 0 ? kept : 1)))) return false;
     csr_compact_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, vals.ptr,
-                                                               new_ptr.ptr, cols2.ptr, vals2.ptr);
+                                                               new_ptr.ptr, cols2.ptr, vals2.ptr, width);
     LSAMD_HIP(hipGetLastError());
     int64_t const kk = static_cast<int64_t>(kept);
     LSAMD_HIP(hipMemcpyAsync(new_ptr.ptr + n, &kk, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
@@ -548,6 +562,20 @@ bool operator_build_csr(OperatorCore &op, int precision) {
   c.nnz = static_cast<int64_t>(kept);
   c.row_ptr = std::move(row_ptr);
   c.cols = std::move(cols);
+  if (cplx) {
+    // complex values (re, im) in FP64 and FP32 for the embedded products
+    int64_t const m = 2 * c.nnz;
+    if (!c.vals32.allocate(static_cast<size_t>(m > 0 ? m : 1))) return false;
+    if (!c.diag32.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
+    if (m > 0) to_f32_kernel<<<grid_for(m, 256), 256, 0, stream()>>>(m, vals.ptr, c.vals32.ptr);
+    if (n > 0) to_f32_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, c.diag64.ptr, c.diag32.ptr);
+    LSAMD_HIP(hipGetLastError());
+    c.vals64 = std::move(vals);
+    c.has32 = c.has64 = true;
+    c.is_complex = true;
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    return true;
+  }
   if (precision == 32 || precision == 96) {
     if (!c.vals32.allocate(static_cast<size_t>(c.nnz > 0 ? c.nnz : 1))) return false;
     if (!c.diag32.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;

@@ -32,6 +32,7 @@ struct Csr {
   DeviceArray<double> diag64;
   DeviceArray<float> diag32;
   bool has64 = false, has32 = false;
+  bool is_complex = false; // values are (re, im) pairs of M[i, j]
 };
 
 struct OperatorCore {

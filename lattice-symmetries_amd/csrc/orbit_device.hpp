@@ -196,7 +196,7 @@ __device__ __forceinline__ StateInfo state_info_one(GroupView const &g,
     out.c_im = cim;
   }
   double const denom = (SI == 0 || kFermi) ? static_cast<double>(G) : static_cast<double>(2 * G);
-  out.norm = sqrt(nn / denom);
+  out.norm = (g.norm_eps > 0.0 && !(nn > g.norm_eps)) ? 0.0 : sqrt(nn / denom);
   return out;
 }
 
@@ -265,7 +265,7 @@ __device__ __forceinline__ void is_representative_one(GroupView const &g,
     flag = flag && f;
   }
   norm_out = total;
-  flag_out = (total > 0) ? static_cast<uint8_t>(flag) : static_cast<uint8_t>(0);
+  flag_out = (total > g.norm_eps) ? static_cast<uint8_t>(flag) : static_cast<uint8_t>(0);
 }
 
 // ---------------------------------------------------------------------------

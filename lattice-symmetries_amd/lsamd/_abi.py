@@ -195,6 +195,7 @@ SIGNATURES = {
     "ls_amd_basis_create": (_P, [C.c_int, C.c_int, C.c_int, C.POINTER(ls_hs_permutation_group)]),
     "ls_amd_fermion_basis_create": (_P, [C.c_int, C.c_int, C.c_int, C.POINTER(ls_hs_permutation_group)]),
     "ls_amd_basis_destroy": (None, [_P]),
+    "ls_amd_basis_set_norm_tolerance": (C.c_int, [_P, C.c_double]),
     "ls_amd_basis_min_state": (C.c_uint64, [_P]),
     "ls_amd_basis_max_state": (C.c_uint64, [_P]),
     "ls_amd_basis_build": (C.c_int64, [_P, C.c_uint64, C.c_uint64]),

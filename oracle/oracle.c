@@ -275,6 +275,8 @@ typedef struct ora_group {
      2 = spinful fermions, sites 0..n/2-1 up and n/2..n-1 down. */
   int fermion_species;
   int32_t *perms; /* size x number_bits, kept for the signs */
+  /* Opt-in: stabiliser sums <= norm_eps count as zero (0: reference rule). */
+  double norm_eps;
 } ora_group;
 
 /* kernels.c:96-98 */
@@ -395,6 +397,8 @@ ORA_EXPORT int ora_group_set_fermionic(ora_group *g, int species) {
   return 0;
 }
 
+ORA_EXPORT void ora_group_set_norm_eps(ora_group *g, double eps) { g->norm_eps = eps; }
+
 ORA_EXPORT int ora_fermion_sign(ora_group const *g, int e, uint64_t x) {
   return fermion_sign(g, e, x);
 }
@@ -450,7 +454,7 @@ static void is_rep_one(ora_group const *g, int lanes, uint64_t x,
     flag = flag && tf;
   }
   *norm_out = total;
-  *flag_out = (total > 0) ? (uint8_t)flag : 0;
+  *flag_out = (total > g->norm_eps) ? (uint8_t)flag : 0;
 }
 
 ORA_EXPORT void ora_is_representative(ora_group const *g, int lanes,
@@ -525,7 +529,7 @@ static void state_info_one(ora_group const *g, int lanes, uint64_t x,
   chr[0] = cur.c_re;
   chr[1] = cur.c_im;
   double const denom = (g->spin_inversion == 0) ? (double)G : (double)(2 * G);
-  *norm = sqrt(cur.n / denom);
+  *norm = (g->norm_eps > 0 && !(cur.n > g->norm_eps)) ? 0.0 : sqrt(cur.n / denom);
 }
 
 ORA_EXPORT void ora_state_info(ora_group const *g, int lanes, ptrdiff_t batch,

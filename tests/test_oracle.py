@@ -226,6 +226,29 @@ def test_numerically_zero_norm_halts_like_the_reference():
         O.matvec_push(g, n, 0, reps, d, o, np.ones(reps.size))
 
 
+def test_norm_tolerance_recovers_every_momentum_sector():
+    """With the opt-in tolerance (stabiliser sums <= 1e-9 count as zero) the
+    reference formula reproduces the numpy-projector spectra of all 12
+    momenta of the 12-site J1-J2 chain, including the complex ones."""
+    n, h = 12, 6
+    expr = models.j1j2_chain(n, 1.0, 0.4)
+    diag, off = expr.split()
+    full_states = H.weight_states(n, h)
+    Mfull = O.matrix_dense(None, n, 0, full_states, O.Terms(diag), O.Terms(off))
+    O.lib().ora_group_set_norm_eps.argtypes = [O.C.c_void_p, O.C.c_double]
+    for k in range(n):
+        perms, num, den, re, im = O.group_closure(n, [models.chain_translation(n)], [k])
+        g = O.Group(n, perms, re, im, 0)
+        O.lib().ora_group_set_norm_eps(g.h, 1e-9)
+        lo = (1 << h) - 1
+        reps = g.enumerate(lo, lo << (n - h))
+        M = O.matrix_dense(g, n, 0, reps, O.Terms(diag), O.Terms(off))
+        Ms = H.sector_matrix_numpy(n, perms, re + 1j * im, 0, Mfull, full_states)
+        assert reps.size == Ms.shape[0]
+        assert np.abs(M - M.conj().T).max() < 1e-12
+        assert np.allclose(np.linalg.eigvalsh(M), np.linalg.eigvalsh(Ms), atol=1e-10)
+
+
 def test_push_matvec_is_real_part_of_projected_matrix():
     n, h = 12, 6
     expr = models.j1j2_chain(n, 1.0, 0.5)
