@@ -283,10 +283,13 @@ int ls_amd_matvec_c128(ls_amd_operator *op, double const *d_x, double *d_y);
 
 /* CSR cache: off-diagonal part as rows of (column, value) plus a dense
    diagonal.  precision: 64 stores FP64 values, 32 stores FP32 values, 96
-   stores both.  Returns the number of stored off-diagonal nonzeros or -1. */
+   stores both; 128 stores complex values M[i, j] as (re, im) pairs in FP64
+   and FP32, for sector batches (the real-valued CSR products reject it).
+   Returns the number of stored off-diagonal nonzeros or -1. */
 int64_t ls_amd_operator_build_csr(ls_amd_operator *op, int precision);
 int64_t ls_amd_operator_csr_nnz(ls_amd_operator const *op);
-/* Stored CSR (off-diagonal) rows: row_ptr[n+1] (int64), cols, values. */
+/* Stored CSR (off-diagonal) rows: row_ptr[n+1] (int64), cols, values
+   (2 nnz interleaved re, im for a complex CSR). */
 int ls_amd_operator_copy_csr(ls_amd_operator const *op, int64_t *row_ptr,
                              uint32_t *cols, double *values, double *diag);
 /* y = H x from the cached CSR: FP64 one vector. */
@@ -319,7 +322,10 @@ typedef struct ls_amd_lanczos_result {
 } ls_amd_lanczos_result;
 
 /* Single-vector Lanczos with full re-orthogonalisation.  mode: 0 FP64 CSR,
-   1 FP32 CSR (FP32 vectors in the product only), 2 on-the-fly FP64.
+   1 FP32 CSR (FP32 vectors in the product only), 2 on-the-fly FP64, 3
+   on-the-fly FP64 with complex vectors (Krylov vectors are the real
+   embedding, 2 n doubles, interleaved re, im; a single start vector sees
+   each eigenvalue of the complex matrix once).
    Seeded N(0,1) start vector.  Stops when the k lowest Ritz pairs have
    relative residual bound < tol or after max_steps.  refine_steps > 0 runs
    that many FP64 Lanczos steps restarted from the lowest Ritz vector.
@@ -334,9 +340,105 @@ int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_size,
                          int refine_steps, double *d_ritz_out,
                          ls_amd_lanczos_result *out);
 
+/* ---- Many symmetry sectors in one product --------------------------------
+   No counterpart in the reference, which builds one basis per sector and
+   diagonalises the sectors one after another (chapel/src/Diagonalize.chpl:
+   134-162).  A sector batch concatenates the complex CSR caches of several
+   operators (each built with ls_amd_operator_build_csr(op, 128)) into one
+   block-diagonal matrix.  Vectors hold k complex right-hand sides per state,
+   laid out [state][re | im][k] over all sectors in order (sector s starts at
+   state offsets[s]).  Read as [2 state + part][k] this is the real
+   embedding [[A, -B], [B, A]] of each sector's M = A + iB.  The product is
+   one persistent launch over every sector, captured in a HIP graph on first
+   use of a given (x, y, k) and replayed afterwards. */
+typedef struct ls_amd_sector_batch ls_amd_sector_batch;
+ls_amd_sector_batch *ls_amd_sector_batch_create(ls_amd_operator *const *ops,
+                                                int count);
+void ls_amd_sector_batch_destroy(ls_amd_sector_batch *batch);
+int ls_amd_sector_batch_count(ls_amd_sector_batch const *batch);
+int64_t ls_amd_sector_batch_number_states(ls_amd_sector_batch const *batch);
+int64_t ls_amd_sector_batch_nnz(ls_amd_sector_batch const *batch);
+/* offsets[count + 1] (host): first state of every sector. */
+int ls_amd_sector_batch_offsets(ls_amd_sector_batch const *batch,
+                                int64_t *offsets);
+/* Product schedule: lanes per row (1, 2, 4, 8, 16), workgroups in the
+   persistent grid, and whether each die's workgroups own one contiguous
+   range of rows (die_aware = 1, grid a multiple of 8). */
+int ls_amd_sector_batch_set_schedule(ls_amd_sector_batch *batch, int lanes,
+                                     int blocks, int die_aware);
+int ls_amd_sector_batch_get_schedule(ls_amd_sector_batch const *batch,
+                                     int *lanes, int *blocks, int *die_aware);
+/* Times every schedule for the FP32 product with k columns and keeps the
+   fastest. */
+int ls_amd_sector_batch_autotune(ls_amd_sector_batch *batch, int k);
+/* y = M x on every sector, k in {1, 2, 4, 8} complex columns. */
+int ls_amd_sector_batch_matvec_f32(ls_amd_sector_batch *batch, int k,
+                                   float const *d_x, float *d_y);
+int ls_amd_sector_batch_matvec_f64(ls_amd_sector_batch *batch, int k,
+                                   double const *d_x, double *d_y);
+
+typedef struct ls_amd_sector_eig_info {
+  int cycles;            /* restart cycles run */
+  int fp64_cycles;       /* of which with the FP64 product */
+  int blocks;            /* block products in the Krylov cycles */
+  int converged;         /* 1 if every sector met tol */
+  int64_t products;      /* all block products, the final check included */
+  double product_seconds;
+  double total_seconds;
+  double max_residual;   /* largest FP64 relative residual reported */
+} ls_amd_sector_eig_info;
+
+/* Thick-restart block Lanczos on every sector at once, on the real
+   embedding with block_size (1, 2, 4, 8) columns; nev eigenvalues per
+   sector (2 nev <= block_size).  max_blocks Krylov blocks per cycle, the
+   lowest keep_blocks Ritz blocks kept at a restart.  The product is FP32
+   (FP64 Krylov vectors and orthogonalisation) until the relative residual
+   falls below switch_tol or stalls, then FP64 until tol.  evals[count *
+   nev] receives each sector's lowest eigenvalues ascending; residuals (may
+   be NULL) their FP64 relative residual norms || M u - lambda u || /
+   max(1, |lambda|), computed with the FP64 product at the end. */
+int ls_amd_sector_batch_eigensolve(ls_amd_sector_batch *batch, int block_size,
+                                   int nev, int max_blocks, int keep_blocks,
+                                   int max_cycles, double tol,
+                                   double switch_tol, uint64_t seed,
+                                   double *evals, double *residuals,
+                                   ls_amd_sector_eig_info *info);
+
+/* ---- Model files --------------------------------------------------------
+   The reference's YAML model format (chapel/data/<model>.yaml: basis +
+   hamiltonian + observables, Yaml.hs:14-48; expression grammar of
+   Parser.hs:60-170 with "sites" replication, Expr.hs:266-286), read without
+   Haskell.  text is the file's content (UTF-8). */
+typedef struct ls_amd_model ls_amd_model;
+ls_amd_model *ls_amd_model_from_yaml(char const *text);
+void ls_amd_model_destroy(ls_amd_model *model);
+/* info[10]: particle (0 spin, 1 spinless fermion, 2 spinful fermion),
+   number of sites, Hamming weight (-1 none), spin inversion (0, +1, -1),
+   number up (or spinless particles; -1 none), number down (-1 none), total
+   spinful particles (-1 none), number of symmetries, has hamiltonian,
+   number of observables. */
+int ls_amd_model_basis_info(ls_amd_model const *model, int *info);
+/* Symmetry `index`: permutation[number_sites] and sector. */
+int ls_amd_model_symmetry(ls_amd_model const *model, int index,
+                          int32_t *permutation, int32_t *sector);
+/* Terms of an operator: which = -1 the hamiltonian, k >= 0 observable k.
+   diagonal selects the terms with x = 0.  Terms are sorted by
+   (m, l, r, x, s); v holds (re, im) pairs. */
+int64_t ls_amd_model_number_terms(ls_amd_model const *model, int which,
+                                  int diagonal);
+int ls_amd_model_terms(ls_amd_model const *model, int which, int diagonal,
+                       double *v, uint64_t *m, uint64_t *l, uint64_t *r,
+                       uint64_t *x, uint64_t *s);
+/* Device basis of the model (not yet built) and an operator on it. */
+ls_amd_basis *ls_amd_model_create_basis(ls_amd_model const *model);
+ls_amd_operator *ls_amd_model_create_operator(ls_amd_model const *model,
+                                              int which,
+                                              ls_amd_basis const *basis);
+
 /* Timing of the last launched kernel of the given family, in milliseconds
    (HIP events on the engine's stream). family: 0 csr f64, 1 csr f32 block,
-   2 on-the-fly f64, 3 enumeration, 4 csr build, 5 csr f64 block. */
+   2 on-the-fly f64, 3 enumeration, 4 csr build, 5 csr f64 block, 6 sector
+   batch product (graph replay). */
 double ls_amd_last_kernel_ms(int family);
 
 #ifdef __cplusplus

@@ -740,9 +740,10 @@ extern "C" int ls_amd_operator_copy_csr(ls_amd_operator const *op, int64_t *row_
                                         double *values, double *diag) {
   Csr const &c = op->core.csr;
   if (!c.has64) return fail("FP64 CSR has not been built") ? 0 : -1;
+  size_t const width = c.is_complex ? 2 : 1;
   bool const ok = to_host(row_ptr, c.row_ptr.ptr, static_cast<size_t>(c.rows + 1)) &&
                   to_host(cols, c.cols.ptr, static_cast<size_t>(c.nnz)) &&
-                  to_host(values, c.vals64.ptr, static_cast<size_t>(c.nnz)) &&
+                  to_host(values, c.vals64.ptr, width * static_cast<size_t>(c.nnz)) &&
                   to_host(diag, c.diag64.ptr, static_cast<size_t>(c.rows));
   return ok ? 0 : -1;
 }

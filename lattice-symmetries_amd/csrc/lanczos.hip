@@ -250,10 +250,16 @@ double seconds_since(Clock::time_point t) {
   return std::chrono::duration<double>(Clock::now() - t).count();
 }
 
+// Length of the Krylov vectors for a product mode: the real embedding of a
+// complex vector (interleaved re, im) for mode 3.
+int64_t vector_length(OperatorCore const &op, int mode) {
+  return mode == 3 ? 2 * op.number_states : op.number_states;
+}
+
 // y = H x in the chosen form (FP64 vectors in and out).
 struct Product {
   OperatorCore *op;
-  int mode; // 0 FP64 CSR, 1 FP32 CSR, 2 on-the-fly FP64
+  int mode; // 0 FP64 CSR, 1 FP32 CSR, 2 on-the-fly FP64, 3 on-the-fly FP64 complex
   DeviceArray<float> xf, yf;
   double seconds = 0.0;
   int64_t calls = 0;
@@ -275,7 +281,7 @@ struct Product {
       ok = operator_csr_f32_block(*op, 1, xf.ptr, yf.ptr);
       f32_to_f64_kernel<<<grid_for(n, kBlock), kBlock, 0, stream()>>>(yf.ptr, y, n);
     } else {
-      ok = operator_matvec_otf(*op, x, y, false, false);
+      ok = operator_matvec_otf(*op, x, y, false, mode == 3);
     }
     LSAMD_HIP(hipStreamSynchronize(stream()));
     seconds += seconds_since(t0);
@@ -292,7 +298,7 @@ struct Product {
 bool lanczos_run(OperatorCore &op, int mode, int max_steps, int k, double tol, uint64_t seed,
                  double const *d_v0, double *d_ritz, std::vector<double> &evals, int &steps,
                  double &matvec_seconds, int64_t &matvec_calls) {
-  int64_t const n = op.number_states;
+  int64_t const n = vector_length(op, mode);
   if (n <= 0) return fail("empty basis");
   int const m_max = static_cast<int>(std::min<int64_t>(max_steps, n));
   DeviceArray<double> V, w, h;
@@ -372,14 +378,16 @@ bool lanczos_run(OperatorCore &op, int mode, int max_steps, int k, double tol, u
   return true;
 }
 
-// FP64 Rayleigh quotient and residual norm of a normalised vector.
-bool rayleigh_residual(OperatorCore &op, double const *d_u, double &theta, double &resid) {
-  int64_t const n = op.number_states;
+// FP64 Rayleigh quotient and residual norm of a normalised vector (the real
+// embedding of a complex vector when complex_vectors).
+bool rayleigh_residual(OperatorCore &op, double const *d_u, double &theta, double &resid, bool complex_vectors) {
+  int64_t const n = complex_vectors ? 2 * op.number_states : op.number_states;
   DeviceArray<double> w;
   Blas blas;
   if (!w.allocate(n) || !blas.init(n, 1)) return false;
-  bool const ok = op.csr.has64 ? operator_csr_f64(op, d_u, w.ptr, false)
-                               : operator_matvec_otf(op, d_u, w.ptr, false, false);
+  bool const ok = (op.csr.has64 && !complex_vectors && !op.csr.is_complex)
+                      ? operator_csr_f64(op, d_u, w.ptr, false)
+                      : operator_matvec_otf(op, d_u, w.ptr, false, complex_vectors);
   if (!ok) return false;
   double uu = 0;
   if (!blas.dot(d_u, d_u, uu) || !blas.dot(d_u, w.ptr, theta)) return false;
@@ -560,11 +568,14 @@ void fill_result(ls_amd_lanczos_result *out, std::vector<double> const &evals, i
 extern "C" int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int k, double tol, uint64_t seed,
                               int refine_steps, double *d_ritz_out, ls_amd_lanczos_result *out) {
   if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
-  if (mode < 0 || mode > 2) return fail("mode must be 0 (FP64 CSR), 1 (FP32 CSR) or 2 (on-the-fly)") ? 0 : -1;
+  if (mode < 0 || mode > 3)
+    return fail("mode must be 0 (FP64 CSR), 1 (FP32 CSR), 2 (on-the-fly) or 3 (on-the-fly, complex)") ? 0 : -1;
   if ((mode == 0 && !op->core.csr.has64) || (mode == 1 && !op->core.csr.has32))
     return fail("build the CSR cache in the requested precision first") ? 0 : -1;
+  if ((mode == 0 || mode == 1) && op->core.csr.is_complex)
+    return fail("the CSR holds complex values: use mode 3") ? 0 : -1;
   auto t0 = Clock::now();
-  int64_t const n = op->core.number_states;
+  int64_t const n = vector_length(op->core, mode);
   DeviceArray<double> u;
   if (!u.allocate(static_cast<size_t>(n))) return -1;
   std::vector<double> evals;
@@ -581,7 +592,7 @@ extern "C" int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int 
     int st2 = 0;
     double mv2 = 0;
     int64_t c2 = 0;
-    int const form = op->core.csr.has64 ? 0 : 2;
+    int const form = mode == 3 ? 3 : (op->core.csr.has64 ? 0 : 2);
     if (!lanczos_run(op->core, form, refine_steps, 1, 0.0, seed, u.ptr, u.ptr, ev2, st2, mv2, c2)) return -1;
     evals[0] = ev2[0];
     mv += mv2;
@@ -589,7 +600,7 @@ extern "C" int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int 
     out->refine_steps = st2;
   }
   double theta = 0, resid = 0;
-  if (!rayleigh_residual(op->core, u.ptr, theta, resid)) return -1;
+  if (!rayleigh_residual(op->core, u.ptr, theta, resid, mode == 3)) return -1;
   fill_result(out, evals, steps, mv, calls, seconds_since(t0));
   out->rayleigh = theta;
   out->residual_norm = resid;
@@ -629,7 +640,7 @@ extern "C" int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_siz
     out->refine_steps = st2;
   }
   double theta = 0, resid = 0;
-  if (!rayleigh_residual(op->core, u.ptr, theta, resid)) return -1;
+  if (!rayleigh_residual(op->core, u.ptr, theta, resid, false)) return -1;
   fill_result(out, evals, blocks, mv, calls, seconds_since(t0));
   out->rayleigh = theta;
   out->residual_norm = resid;

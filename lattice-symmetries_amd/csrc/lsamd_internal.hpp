@@ -48,7 +48,8 @@ enum TimerFamily {
   kTimerEnumerate = 3,
   kTimerCsrBuild = 4,
   kTimerCsrF64Block = 5,
-  kTimerCount = 6
+  kTimerSectorProduct = 6,
+  kTimerCount = 7
 };
 void timer_begin(int family);
 void timer_end(int family);

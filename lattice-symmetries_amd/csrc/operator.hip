@@ -682,6 +682,7 @@ template <typename F> bool dispatch_k(int k, F &&f) {
 
 bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool accumulate) {
   Csr const &c = op.csr;
+  if (c.is_complex) return fail("the CSR holds complex values: use a sector batch for its products");
   if (!c.has64) return fail("FP64 CSR has not been built");
   int64_t const n = c.rows;
   if (n == 0) return true;
@@ -699,6 +700,7 @@ bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool acc
 
 bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_y) {
   Csr const &c = op.csr;
+  if (c.is_complex) return fail("the CSR holds complex values: use a sector batch for its products");
   if (!c.has32) return fail("FP32 CSR has not been built");
   int64_t const n = c.rows;
   if (n == 0) return true;
@@ -717,6 +719,7 @@ bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_
 
 bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *d_y) {
   Csr const &c = op.csr;
+  if (c.is_complex) return fail("the CSR holds complex values: use a sector batch for its products");
   if (!c.has64) return fail("FP64 CSR has not been built");
   int64_t const n = c.rows;
   if (n == 0) return true;
@@ -738,7 +741,7 @@ bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *
 int operator_autotune(OperatorCore &op) {
   Csr const &c = op.csr;
   int64_t const n = c.rows;
-  if (n == 0) return op.lanes_f64;
+  if (n == 0 || c.is_complex) return op.lanes_f64;
   int const cand[] = {2, 4, 8, 16, 32, 64};
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);

@@ -129,6 +129,19 @@ class ls_amd_lanczos_result(C.Structure):
     ]
 
 
+class ls_amd_sector_eig_info(C.Structure):
+    _fields_ = [
+        ("cycles", C.c_int),
+        ("fp64_cycles", C.c_int),
+        ("blocks", C.c_int),
+        ("converged", C.c_int),
+        ("products", C.c_int64),
+        ("product_seconds", C.c_double),
+        ("total_seconds", C.c_double),
+        ("max_residual", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _u64p = C.POINTER(C.c_uint64)
 _i64p = C.POINTER(C.c_int64)
@@ -223,6 +236,21 @@ SIGNATURES = {
                                  C.POINTER(ls_amd_lanczos_result)]),
     "ls_amd_block_lanczos": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int,
                                        _P, C.POINTER(ls_amd_lanczos_result)]),
+    "ls_amd_sector_batch_create": (_P, [C.POINTER(_P), C.c_int]),
+    "ls_amd_sector_batch_destroy": (None, [_P]),
+    "ls_amd_sector_batch_count": (C.c_int, [_P]),
+    "ls_amd_sector_batch_number_states": (C.c_int64, [_P]),
+    "ls_amd_sector_batch_nnz": (C.c_int64, [_P]),
+    "ls_amd_sector_batch_offsets": (C.c_int, [_P, _i64p]),
+    "ls_amd_sector_batch_set_schedule": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "ls_amd_sector_batch_get_schedule": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                   C.POINTER(C.c_int)]),
+    "ls_amd_sector_batch_autotune": (C.c_int, [_P, C.c_int]),
+    "ls_amd_sector_batch_matvec_f32": (C.c_int, [_P, C.c_int, _P, _P]),
+    "ls_amd_sector_batch_matvec_f64": (C.c_int, [_P, C.c_int, _P, _P]),
+    "ls_amd_sector_batch_eigensolve": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                 C.c_double, C.c_uint64, _f64p, _f64p,
+                                                 C.POINTER(ls_amd_sector_eig_info)]),
 }
 
 
