@@ -333,12 +333,42 @@ typedef struct ls_amd_lanczos_result {
 int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int k,
                    double tol, uint64_t seed, int refine_steps,
                    double *d_ritz_out, ls_amd_lanczos_result *out);
-/* Block Lanczos with block_size (<= 16) right-hand sides per product.
-   mode: 0 FP64 block CSR, 1 FP32 block CSR. */
+/* Block Lanczos with block_size (1, 2, 4, 8) right-hand sides per product,
+   through ls_amd_block_eigensolve: max_blocks Krylov blocks per restart
+   cycle, the k lowest eigenvalues (k <= block_size).  mode: 0 FP64 block
+   CSR, 1 FP32 block CSR until the FP32 residual floor, then FP64 (needs the
+   CSR in both precisions).  refine_steps > 0 adds an FP64 Lanczos run
+   restarted from the lowest Ritz vector. */
 int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_size,
                          int max_blocks, int k, double tol, uint64_t seed,
                          int refine_steps, double *d_ritz_out,
                          ls_amd_lanczos_result *out);
+
+typedef struct ls_amd_eig_info {
+  int cycles;            /* restart cycles run */
+  int fp64_cycles;       /* of which with the FP64 product */
+  int blocks;            /* block products in the Krylov cycles */
+  int converged;         /* 1 if every segment met tol */
+  int64_t products;      /* all block products, the final check included */
+  double product_seconds;
+  double total_seconds;
+  double max_residual;   /* largest FP64 relative residual reported */
+} ls_amd_eig_info;
+
+/* Thick-restart block Lanczos on one operator (FP64 Krylov blocks and
+   orthogonalisation, two passes of full re-orthogonalisation, CholQR2).
+   block_size in {1, 2, 4, 8}; nev <= block_size eigenvalues; max_blocks
+   Krylov blocks per cycle; keep_blocks Ritz blocks kept at a restart.
+   mode 1: FP32 block product until the relative residual falls below
+   switch_tol (or stalls), then FP64 until tol; mode 0: FP64 throughout.
+   evals[nev] ascending; residuals[nev] (may be NULL) are FP64 relative
+   residual norms || H u - lambda u || / max(1, |lambda|); d_ritz_out (device,
+   may be NULL) receives the lowest Ritz vector. */
+int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size,
+                            int nev, int max_blocks, int keep_blocks,
+                            int max_cycles, double tol, double switch_tol,
+                            uint64_t seed, double *evals, double *residuals,
+                            double *d_ritz_out, ls_amd_eig_info *info);
 
 /* ---- Many symmetry sectors in one product --------------------------------
    No counterpart in the reference, which builds one basis per sector and
@@ -377,18 +407,8 @@ int ls_amd_sector_batch_matvec_f32(ls_amd_sector_batch *batch, int k,
 int ls_amd_sector_batch_matvec_f64(ls_amd_sector_batch *batch, int k,
                                    double const *d_x, double *d_y);
 
-typedef struct ls_amd_sector_eig_info {
-  int cycles;            /* restart cycles run */
-  int fp64_cycles;       /* of which with the FP64 product */
-  int blocks;            /* block products in the Krylov cycles */
-  int converged;         /* 1 if every sector met tol */
-  int64_t products;      /* all block products, the final check included */
-  double product_seconds;
-  double total_seconds;
-  double max_residual;   /* largest FP64 relative residual reported */
-} ls_amd_sector_eig_info;
-
-/* Thick-restart block Lanczos on every sector at once, on the real
+/* Thick-restart block Lanczos (as ls_amd_block_eigensolve) on every sector
+   at once, one product launch per block step, on the real
    embedding with block_size (1, 2, 4, 8) columns; nev eigenvalues per
    sector (2 nev <= block_size).  max_blocks Krylov blocks per cycle, the
    lowest keep_blocks Ritz blocks kept at a restart.  The product is FP32
@@ -402,7 +422,7 @@ int ls_amd_sector_batch_eigensolve(ls_amd_sector_batch *batch, int block_size,
                                    int max_cycles, double tol,
                                    double switch_tol, uint64_t seed,
                                    double *evals, double *residuals,
-                                   ls_amd_sector_eig_info *info);
+                                   ls_amd_eig_info *info);
 
 /* ---- Model files --------------------------------------------------------
    The reference's YAML model format (chapel/data/<model>.yaml: basis +

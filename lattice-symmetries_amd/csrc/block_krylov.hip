@@ -1,0 +1,628 @@
+// block_krylov.hip -- thick-restart block Lanczos on FP64 row-major blocks,
+// for one operator or for many independent segments (symmetry sectors) at
+// once.
+//
+// The consumer of the product: the reference drives its product from scipy
+// eigsh (python/lattice_symmetries/__init__.py:591-752) or PRIMME
+// (chapel/src/Diagonalize.chpl:134-162), one sector at a time.  Here every
+// segment runs its own Krylov process but they share each product launch
+// and each reduction launch.
+//
+// Blocks are [rows][B] row-major FP64.  Per block step: one product (FP32
+// or FP64, supplied by the caller), two passes of full re-orthogonalisation
+// against every stored block by segmented Gram reductions (fixed-order
+// two-pass sums) and block updates, CholQR2 per segment, and the projected
+// matrix filled from the re-orthogonalisation coefficients.  At the end of a
+// cycle each segment's projected matrix is diagonalised on a host thread and
+// the lowest `keep` Ritz blocks plus the residual block restart the next
+// cycle.  The product runs in FP32 until the relative residual falls below
+// switch_tol (or stops improving), then the process restarts explicitly from
+// the lowest Ritz block with the FP64 product.  Final eigenvalues and
+// residuals are FP64 Rayleigh quotients and residual norms of the Ritz
+// vectors.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <thread>
+#include <vector>
+
+#include "operator_internal.hpp"
+#include "orbit_device.hpp"
+
+namespace lsamd {
+namespace {
+
+constexpr int kThreads = 256;
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+// ---------------------------------------------------------------------------
+// Segmented FP64 block algebra on row-major [rows][B] blocks.  Rows are cut
+// into chunks that never straddle a segment; every reduction is a fixed-order
+// two-pass sum, so results do not depend on scheduling.
+// ---------------------------------------------------------------------------
+
+constexpr int kChunkRows = 2048;
+
+// partial[(c * nblk + i) * B*B + p*B + q] = sum over rows r of chunk c of
+// A_i[r][p] * W[r][q], with A_i = A + i * a_stride.
+template <int B>
+__global__ __launch_bounds__(kThreads) void seg_gram_kernel(double const *__restrict__ A, int64_t a_stride, int nblk,
+                                                            double const *__restrict__ W,
+                                                            int64_t const *__restrict__ chunk_rows,
+                                                            double *__restrict__ partial) {
+  constexpr int V = B * B;
+  __shared__ double s[kThreads / 64][64];
+  int const c = blockIdx.x;
+  int64_t const r0 = chunk_rows[c], r1 = chunk_rows[c + 1];
+  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = 0; i < nblk; ++i) {
+    double const *__restrict__ Ai = A + static_cast<int64_t>(i) * a_stride;
+    double acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kThreads) {
+      double a[B], w[B];
+#pragma unroll
+      for (int p = 0; p < B; ++p) {
+        a[p] = Ai[r * B + p];
+        w[p] = W[r * B + p];
+      }
+#pragma unroll
+      for (int p = 0; p < B; ++p)
+#pragma unroll
+        for (int q = 0; q < B; ++q) acc[p * B + q] = fma(a[p], w[q], acc[p * B + q]);
+    }
+    // Transpose-reduce across the wave: after the steps below lane L holds
+    // the wave's sum for value index L & (V - 1).
+#pragma unroll
+    for (int h = V / 2; h > 0; h >>= 1) {
+      bool const upper = (lane & h) != 0;
+#pragma unroll
+      for (int v = 0; v < h; ++v) {
+        double const send = upper ? acc[v] : acc[v + h];
+        double const keep = upper ? acc[v + h] : acc[v];
+        acc[v] = keep + __shfl_xor(send, h, 64);
+      }
+    }
+    double tot = acc[0];
+    for (int off = V; off < 64; off <<= 1) tot += __shfl_xor(tot, off, 64);
+    s[wave][lane] = tot;
+    __syncthreads();
+    if (threadIdx.x < V) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; ++w) sum += s[w][threadIdx.x];
+      partial[(static_cast<int64_t>(c) * nblk + i) * V + threadIdx.x] = sum;
+    }
+    __syncthreads();
+  }
+}
+
+// out[(s * nblk + i) * V + t] = sum over the chunks of segment s, in order.
+__global__ __launch_bounds__(64) void seg_reduce_kernel(double const *__restrict__ partial, int nblk, int V,
+                                                        int const *__restrict__ seg_chunk,
+                                                        double *__restrict__ out) {
+  int const s = blockIdx.x, i = blockIdx.y, t = threadIdx.x;
+  if (t >= V) return;
+  double sum = 0.0;
+  for (int c = seg_chunk[s]; c < seg_chunk[s + 1]; ++c) sum += partial[(static_cast<int64_t>(c) * nblk + i) * V + t];
+  out[(static_cast<int64_t>(s) * nblk + i) * V + t] = sum;
+}
+
+// out[r][q] = beta * out[r][q] + sum_{i < nblk} sum_p A_i[r][p] C_s[i][p][q]
+// for the segment s of row r.  Each thread owns whole rows, so out may alias
+// one of the A_i.
+template <int B>
+__global__ __launch_bounds__(kThreads) void seg_combine_kernel(double *out, double beta, double const *A,
+                                                               int64_t a_stride, int nblk,
+                                                               double const *__restrict__ coef,
+                                                               int64_t const *__restrict__ chunk_rows,
+                                                               int const *__restrict__ chunk_seg) {
+  constexpr int V = B * B;
+  extern __shared__ double sc[];
+  int const c = blockIdx.x;
+  int const s = chunk_seg[c];
+  int64_t const r0 = chunk_rows[c], r1 = chunk_rows[c + 1];
+  double const *__restrict__ cs = coef + static_cast<int64_t>(s) * nblk * V;
+  for (int t = threadIdx.x; t < nblk * V; t += kThreads) sc[t] = cs[t];
+  __syncthreads();
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kThreads) {
+    double acc[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) acc[q] = 0.0;
+    for (int i = 0; i < nblk; ++i) {
+      double a[B];
+#pragma unroll
+      for (int p = 0; p < B; ++p) a[p] = A[static_cast<int64_t>(i) * a_stride + r * B + p];
+#pragma unroll
+      for (int p = 0; p < B; ++p)
+#pragma unroll
+        for (int q = 0; q < B; ++q) acc[q] = fma(a[p], sc[i * V + p * B + q], acc[q]);
+    }
+    if (beta != 0.0) {
+#pragma unroll
+      for (int q = 0; q < B; ++q) acc[q] = fma(beta, out[r * B + q], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) out[r * B + q] = acc[q];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void to_f32_rows_kernel(double const *__restrict__ x, float *__restrict__ y,
+                                                               int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) y[i] = static_cast<float>(x[i]);
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Seeded N(0, 1) entries (splitmix64 of (seed, index), Box-Muller).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void normal_kernel(uint64_t seed, T *__restrict__ x, int64_t n) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  uint64_t const a = mix64(seed ^ mix64(2 * static_cast<uint64_t>(i)));
+  uint64_t const b = mix64(seed ^ mix64(2 * static_cast<uint64_t>(i) + 1));
+  double const u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+  double const u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+  x[i] = static_cast<T>(sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2));
+}
+
+template <typename F> bool dispatch_block(int b, F &&f) {
+  switch (b) {
+  case 1: return f.template operator()<1>();
+  case 2: return f.template operator()<2>();
+  case 4: return f.template operator()<4>();
+  case 8: return f.template operator()<8>();
+  default: return fail("block size must be 1, 2, 4 or 8");
+  }
+}
+
+// Chunks of the rows, per segment.
+struct Segments {
+  int nseg = 0;
+  int64_t rows = 0;
+  int chunks = 0;
+  std::vector<int64_t> row_begin; // first row of every segment, nseg + 1
+  DeviceArray<int64_t> d_chunk_rows;
+  DeviceArray<int> d_chunk_seg, d_seg_chunk;
+  bool init(std::vector<int64_t> const &seg_rows) {
+    nseg = static_cast<int>(seg_rows.size()) - 1;
+    row_begin = seg_rows;
+    rows = row_begin[nseg];
+    std::vector<int64_t> cr{0};
+    std::vector<int> cs, sc{0};
+    for (int s = 0; s < nseg; ++s) {
+      for (int64_t r = row_begin[s]; r < row_begin[s + 1]; r += kChunkRows) {
+        cr.push_back(std::min(r + kChunkRows, row_begin[s + 1]));
+        cs.push_back(s);
+      }
+      sc.push_back(static_cast<int>(cs.size()));
+    }
+    chunks = static_cast<int>(cs.size());
+    return d_chunk_rows.upload(cr.data(), cr.size()) && d_chunk_seg.upload(cs.data(), std::max<size_t>(1, cs.size())) &&
+           d_seg_chunk.upload(sc.data(), sc.size());
+  }
+};
+
+// Host-side dense helpers for the per-segment small matrices.
+// Cholesky G = L L^T of a B x B SPD matrix; returns false if a pivot is not
+// positive (rank deficiency).  R = L^T (upper), Rinv = R^{-1}.
+bool cholesky_upper(std::vector<double> const &G, int B, std::vector<double> &R, std::vector<double> &Rinv) {
+  std::vector<double> L(static_cast<size_t>(B) * B, 0.0);
+  double scale = 0.0;
+  for (int i = 0; i < B; ++i) scale = std::max(scale, G[static_cast<size_t>(i) * B + i]);
+  for (int j = 0; j < B; ++j) {
+    double d = G[static_cast<size_t>(j) * B + j];
+    for (int k = 0; k < j; ++k) d -= L[static_cast<size_t>(j) * B + k] * L[static_cast<size_t>(j) * B + k];
+    if (!(d > 1e-28 * std::max(scale, 1e-300))) return false;
+    double const ljj = std::sqrt(d);
+    L[static_cast<size_t>(j) * B + j] = ljj;
+    for (int i = j + 1; i < B; ++i) {
+      double v = G[static_cast<size_t>(i) * B + j];
+      for (int k = 0; k < j; ++k) v -= L[static_cast<size_t>(i) * B + k] * L[static_cast<size_t>(j) * B + k];
+      L[static_cast<size_t>(i) * B + j] = v / ljj;
+    }
+  }
+  R.assign(static_cast<size_t>(B) * B, 0.0);
+  for (int i = 0; i < B; ++i)
+    for (int j = 0; j < B; ++j) R[static_cast<size_t>(i) * B + j] = L[static_cast<size_t>(j) * B + i];
+  // Rinv by back substitution, column by column
+  Rinv.assign(static_cast<size_t>(B) * B, 0.0);
+  for (int col = 0; col < B; ++col) {
+    for (int i = B - 1; i >= 0; --i) {
+      double v = (i == col) ? 1.0 : 0.0;
+      for (int k = i + 1; k < B; ++k) v -= R[static_cast<size_t>(i) * B + k] * Rinv[static_cast<size_t>(k) * B + col];
+      Rinv[static_cast<size_t>(i) * B + col] = v / R[static_cast<size_t>(i) * B + i];
+    }
+  }
+  return true;
+}
+
+// C = A * B for B x B row-major matrices.
+std::vector<double> matmul(std::vector<double> const &A, std::vector<double> const &Bm, int B) {
+  std::vector<double> C(static_cast<size_t>(B) * B, 0.0);
+  for (int i = 0; i < B; ++i)
+    for (int k = 0; k < B; ++k)
+      for (int j = 0; j < B; ++j)
+        C[static_cast<size_t>(i) * B + j] += A[static_cast<size_t>(i) * B + k] * Bm[static_cast<size_t>(k) * B + j];
+  return C;
+}
+
+template <typename F> void parallel_for(int count, F &&f) {
+  unsigned const hw = std::max(1u, std::thread::hardware_concurrency());
+  int const nt = static_cast<int>(std::min<unsigned>(std::min<unsigned>(hw, 16u), static_cast<unsigned>(count)));
+  if (nt <= 1) {
+    for (int i = 0; i < count; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      for (int i = t; i < count; i += nt) f(i);
+    });
+  for (auto &th : pool) th.join();
+}
+
+struct BlockAlgebra {
+  Segments seg;
+  int B = 8;
+  DeviceArray<double> partial, d_out, d_coef;
+  std::vector<double> h_coef;
+  int max_nblk = 0;
+  bool init(std::vector<int64_t> const &offsets, int b, int nblk_max) {
+    B = b;
+    max_nblk = nblk_max;
+    if (!seg.init(offsets)) return false;
+    size_t const V = static_cast<size_t>(B) * B;
+    return partial.allocate(std::max<size_t>(1, static_cast<size_t>(seg.chunks) * nblk_max * V)) &&
+           d_out.allocate(static_cast<size_t>(seg.nseg) * nblk_max * V) &&
+           d_coef.allocate(static_cast<size_t>(seg.nseg) * nblk_max * V);
+  }
+  // out_host[(s * nblk + i) * V + p * B + q] = A_i^T W over segment s
+  bool gram(double const *A, int64_t a_stride, int nblk, double const *W, std::vector<double> &out_host) {
+    if (nblk > max_nblk) return fail("internal: too many blocks in a Gram product");
+    int const V = B * B;
+    if (seg.chunks > 0) {
+      bool ok = dispatch_block(B, [&]<int BB>() -> bool {
+        seg_gram_kernel<BB><<<seg.chunks, kThreads, 0, stream()>>>(A, a_stride, nblk, W, seg.d_chunk_rows.ptr,
+                                                                   partial.ptr);
+        return true;
+      });
+      if (!ok) return false;
+      seg_reduce_kernel<<<dim3(seg.nseg, nblk), 64, 0, stream()>>>(partial.ptr, nblk, V, seg.d_seg_chunk.ptr,
+                                                                    d_out.ptr);
+      LSAMD_HIP(hipGetLastError());
+    }
+    out_host.assign(static_cast<size_t>(seg.nseg) * nblk * V, 0.0);
+    if (seg.chunks > 0)
+      LSAMD_HIP(hipMemcpyAsync(out_host.data(), d_out.ptr, sizeof(double) * out_host.size(), hipMemcpyDeviceToHost,
+                               stream()));
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    return true;
+  }
+  // out = beta * out + A * C (per segment), coefficients [s][i][p][q] on the host
+  bool combine(double *out, double beta, double const *A, int64_t a_stride, int nblk,
+               std::vector<double> const &coef_host) {
+    if (nblk > max_nblk) return fail("internal: too many blocks in a block update");
+    int const V = B * B;
+    if (seg.chunks == 0) return true;
+    h_coef = coef_host;
+    LSAMD_HIP(hipMemcpyAsync(d_coef.ptr, h_coef.data(), sizeof(double) * h_coef.size(), hipMemcpyHostToDevice,
+                             stream()));
+    size_t const smem = sizeof(double) * static_cast<size_t>(nblk) * V;
+    bool ok = dispatch_block(B, [&]<int BB>() -> bool {
+      seg_combine_kernel<BB><<<seg.chunks, kThreads, smem, stream()>>>(out, beta, A, a_stride, nblk, d_coef.ptr,
+                                                                       seg.d_chunk_rows.ptr, seg.d_chunk_seg.ptr);
+      return true;
+    });
+    if (!ok) return false;
+    LSAMD_HIP(hipGetLastError());
+    // the host coefficients must outlive the copy
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    return true;
+  }
+};
+
+
+void jacobi_eigh_local(std::vector<double> A, int m, std::vector<double> &w, std::vector<double> &Z) {
+  Z.assign(static_cast<size_t>(m) * m, 0.0);
+  for (int i = 0; i < m; ++i) Z[static_cast<size_t>(i) * m + i] = 1.0;
+  auto at = [&](int i, int j) -> double & { return A[static_cast<size_t>(i) * m + j]; };
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) {
+        tot += at(i, j) * at(i, j);
+        if (i != j) off += at(i, j) * at(i, j);
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < m - 1; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        double const apq = at(p, q);
+        if (std::fabs(apq) < 1e-300) continue;
+        double const theta = (at(q, q) - at(p, p)) / (2.0 * apq);
+        double const t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double const c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < m; ++k) {
+          double const akp = at(k, p), akq = at(k, q);
+          at(k, p) = c * akp - s * akq;
+          at(k, q) = s * akp + c * akq;
+        }
+        for (int k = 0; k < m; ++k) {
+          double const apk = at(p, k), aqk = at(q, k);
+          at(p, k) = c * apk - s * aqk;
+          at(q, k) = s * apk + c * aqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          double const zkp = Z[static_cast<size_t>(k) * m + p], zkq = Z[static_cast<size_t>(k) * m + q];
+          Z[static_cast<size_t>(k) * m + p] = c * zkp - s * zkq;
+          Z[static_cast<size_t>(k) * m + q] = s * zkp + c * zkq;
+        }
+      }
+  }
+  std::vector<int> order(m);
+  for (int i = 0; i < m; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return at(a, a) < at(b, b); });
+  w.resize(m);
+  std::vector<double> Zs(static_cast<size_t>(m) * m);
+  for (int c = 0; c < m; ++c) {
+    w[c] = at(order[c], order[c]);
+    for (int r = 0; r < m; ++r) Zs[static_cast<size_t>(r) * m + c] = Z[static_cast<size_t>(r) * m + order[c]];
+  }
+  Z.swap(Zs);
+}
+
+__global__ __launch_bounds__(kThreads) void column_kernel(double const *__restrict__ X, int b, int col,
+                                                          double *__restrict__ out, int64_t rows) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < rows) out[i] = X[i * b + col];
+}
+
+} // namespace
+
+bool block_to_f32(double const *x, float *y, int64_t count) {
+  if (count <= 0) return true;
+  to_f32_rows_kernel<<<grid_for(count, kThreads), kThreads, 0, stream()>>>(x, y, count);
+  LSAMD_HIP(hipGetLastError());
+  return true;
+}
+
+bool block_column(double const *X, int b, int col, double *out, int64_t rows) {
+  if (rows <= 0) return true;
+  column_kernel<<<grid_for(rows, kThreads), kThreads, 0, stream()>>>(X, b, col, out, rows);
+  LSAMD_HIP(hipGetLastError());
+  return true;
+}
+
+bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, BlockProduct const &product,
+                        BlockKrylovOptions const &o, BlockKrylovResult &res, double *d_ritz_block) {
+  auto const t_start = Clock::now();
+  int const B = o.block;
+  if (B != 1 && B != 2 && B != 4 && B != 8) return fail("block size must be 1, 2, 4 or 8");
+  int const want = multiplicity * o.nev; // Ritz values per segment
+  if (o.nev < 1 || want > B) return fail("nev (times 2 for complex sectors) must not exceed the block size");
+  if (o.keep < 1 || o.keep >= o.max_blocks - 1) return fail("keep must be in [1, max_blocks - 2]");
+  if (o.max_cycles < 1) return fail("max_cycles must be positive");
+  int const ns = static_cast<int>(seg_rows.size()) - 1;
+  if (ns < 1) return fail("no segments");
+  int const m = o.max_blocks;
+  int const V = B * B;
+  int64_t min_rows = INT64_MAX;
+  for (int s = 0; s < ns; ++s) min_rows = std::min(min_rows, seg_rows[s + 1] - seg_rows[s]);
+  if (min_rows < static_cast<int64_t>(m + 1) * B)
+    return fail("a segment is too small for (max_blocks + 1) * block_size Krylov vectors (" +
+                std::to_string(min_rows) + " rows)");
+  BlockAlgebra alg;
+  if (!alg.init(seg_rows, B, m)) return false;
+  int64_t const rows = alg.seg.rows;
+  int64_t const blk = rows * B; // doubles per block
+  DeviceArray<double> Vb, W, Y;
+  if (!Vb.allocate(static_cast<size_t>(blk) * m) || !W.allocate(static_cast<size_t>(blk)) ||
+      !Y.allocate(static_cast<size_t>(blk) * o.keep))
+    return false;
+  auto Vj = [&](int j) { return Vb.ptr + static_cast<int64_t>(j) * blk; };
+
+  double prod_s = 0.0;
+  int64_t prods = 0;
+  auto apply = [&](bool fp32, double const *x, double *y) -> bool {
+    auto t0 = Clock::now();
+    if (!product(fp32, x, y)) return false;
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    prod_s += seconds_since(t0);
+    ++prods;
+    return true;
+  };
+  std::vector<double> G, Rf(static_cast<size_t>(ns) * V), coef;
+  // CholQR2 of block X into out (may be X); Rf receives the factor R per segment.
+  auto cholqr = [&](double *X, double *out) -> bool {
+    std::vector<double> Racc(static_cast<size_t>(ns) * V, 0.0);
+    for (int s = 0; s < ns; ++s)
+      for (int i = 0; i < B; ++i) Racc[static_cast<size_t>(s) * V + i * B + i] = 1.0;
+    double *src = X;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (!alg.gram(src, blk, 1, src, G)) return false;
+      coef.assign(static_cast<size_t>(ns) * V, 0.0);
+      for (int s = 0; s < ns; ++s) {
+        std::vector<double> Gs(G.begin() + static_cast<int64_t>(s) * V, G.begin() + static_cast<int64_t>(s + 1) * V);
+        std::vector<double> R, Rinv;
+        if (!cholesky_upper(Gs, B, R, Rinv))
+          return fail("block Lanczos breakdown in segment " + std::to_string(s) +
+                      " (the Krylov space is exhausted; lower max_blocks or the block size)");
+        std::copy(Rinv.begin(), Rinv.end(), coef.begin() + static_cast<int64_t>(s) * V);
+        std::vector<double> Ra(Racc.begin() + static_cast<int64_t>(s) * V,
+                               Racc.begin() + static_cast<int64_t>(s + 1) * V);
+        std::vector<double> const Rn = matmul(R, Ra, B);
+        std::copy(Rn.begin(), Rn.end(), Racc.begin() + static_cast<int64_t>(s) * V);
+      }
+      if (!alg.combine(out, 0.0, src, blk, 1, coef)) return false;
+      src = out;
+    }
+    Rf = Racc;
+    return true;
+  };
+
+  // start block: seeded N(0, 1), orthonormalised per segment
+  normal_kernel<double><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, Vj(0), blk);
+  LSAMD_HIP(hipGetLastError());
+  if (!cholqr(Vj(0), Vj(0))) return false;
+
+  int const dimT = m * B;
+  std::vector<std::vector<double>> T(ns, std::vector<double>(static_cast<size_t>(dimT) * dimT, 0.0));
+  std::vector<std::vector<double>> theta(ns), Z(ns);
+  std::vector<double> worst(ns, 0.0);
+  bool fp32 = o.fp32;
+  int kept = 0;
+  int cycles_run = 0, fp64_cycles = 0, blocks_done = 0;
+  double prev_worst = INFINITY;
+  bool done = false;
+  std::vector<double> C, Csum;
+  for (int cycle = 0; cycle < o.max_cycles; ++cycle) {
+    if (!fp32) ++fp64_cycles;
+    for (int j = kept; j < m; ++j) {
+      if (!apply(fp32, Vj(j), W.ptr)) return false;
+      ++blocks_done;
+      int const nb = j + 1;
+      Csum.assign(static_cast<size_t>(ns) * nb * V, 0.0);
+      for (int pass = 0; pass < 2; ++pass) {
+        if (!alg.gram(Vb.ptr, blk, nb, W.ptr, C)) return false;
+        for (size_t t = 0; t < C.size(); ++t) {
+          Csum[t] += C[t];
+          C[t] = -C[t];
+        }
+        if (!alg.combine(W.ptr, 1.0, Vb.ptr, blk, nb, C)) return false;
+      }
+      // column j of the projected matrix (and its mirror) from the coefficients
+      for (int s = 0; s < ns; ++s) {
+        std::vector<double> &Ts = T[s];
+        for (int i = 0; i < nb; ++i) {
+          double const *Ci = Csum.data() + (static_cast<size_t>(s) * nb + i) * V;
+          for (int p = 0; p < B; ++p)
+            for (int q = 0; q < B; ++q) {
+              size_t const r = static_cast<size_t>(i * B + p), c = static_cast<size_t>(j * B + q);
+              if (i == j) {
+                Ts[r * dimT + c] = 0.5 * (Ci[p * B + q] + Ci[q * B + p]);
+              } else {
+                Ts[r * dimT + c] = Ci[p * B + q];
+                Ts[c * dimT + r] = Ci[p * B + q];
+              }
+            }
+        }
+      }
+      if (j + 1 < m) {
+        if (!cholqr(W.ptr, Vj(j + 1))) return false;
+      } else {
+        if (!cholqr(W.ptr, W.ptr)) return false; // residual block; Rf holds its factor
+      }
+    }
+    // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
+    parallel_for(ns, [&](int s) {
+      jacobi_eigh_local(T[s], dimT, theta[s], Z[s]);
+      double w = 0.0;
+      double const *Rs = Rf.data() + static_cast<size_t>(s) * V;
+      for (int q = 0; q < want; ++q) {
+        double rs = 0.0;
+        for (int p = 0; p < B; ++p) {
+          double acc = 0.0;
+          for (int c = 0; c < B; ++c) acc += Rs[p * B + c] * Z[s][static_cast<size_t>((m - 1) * B + c) * dimT + q];
+          rs += acc * acc;
+        }
+        w = std::max(w, std::sqrt(rs) / std::max(1.0, std::fabs(theta[s][q])));
+      }
+      worst[s] = w;
+    });
+    double wmax = 0.0;
+    for (int s = 0; s < ns; ++s) wmax = std::max(wmax, worst[s]);
+    ++cycles_run;
+    res.history.push_back(wmax);
+    if (!fp32 && wmax < o.tol) {
+      done = true;
+      break;
+    }
+    if (cycle + 1 == o.max_cycles) break; // keep V and Z for the final Ritz block
+    // FP32 gives way to FP64 at the switch tolerance or when FP32 stalls
+    bool const to_fp64 = fp32 && (wmax < o.switch_tol || wmax > 0.5 * prev_worst);
+    prev_worst = wmax;
+    // Restart: Y_t = sum_i V_i Z_s[i-block, t-block]
+    int const r = to_fp64 ? 1 : o.keep;
+    for (int t = 0; t < r; ++t) {
+      coef.assign(static_cast<size_t>(ns) * m * V, 0.0);
+      for (int s = 0; s < ns; ++s)
+        for (int i = 0; i < m; ++i)
+          for (int p = 0; p < B; ++p)
+            for (int q = 0; q < B; ++q)
+              coef[(static_cast<size_t>(s) * m + i) * V + p * B + q] =
+                  Z[s][static_cast<size_t>(i * B + p) * dimT + t * B + q];
+      if (!alg.combine(Y.ptr + static_cast<int64_t>(t) * blk, 0.0, Vb.ptr, blk, m, coef)) return false;
+    }
+    LSAMD_HIP(hipMemcpyAsync(Vb.ptr, Y.ptr, sizeof(double) * blk * r, hipMemcpyDeviceToDevice, stream()));
+    for (int s = 0; s < ns; ++s) std::fill(T[s].begin(), T[s].end(), 0.0);
+    if (to_fp64) {
+      // explicit restart from the lowest Ritz block with the FP64 product
+      fp32 = false;
+      kept = 0;
+      if (!cholqr(Vj(0), Vj(0))) return false;
+    } else {
+      LSAMD_HIP(hipMemcpyAsync(Vj(r), W.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
+      for (int s = 0; s < ns; ++s)
+        for (int q = 0; q < r * B; ++q) T[s][static_cast<size_t>(q) * dimT + q] = theta[s][q];
+      kept = r;
+    }
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+  }
+  // Final: the lowest Ritz block, its FP64 Rayleigh quotients and residuals.
+  coef.assign(static_cast<size_t>(ns) * m * V, 0.0);
+  for (int s = 0; s < ns; ++s)
+    for (int i = 0; i < m; ++i)
+      for (int p = 0; p < B; ++p)
+        for (int q = 0; q < B; ++q)
+          coef[(static_cast<size_t>(s) * m + i) * V + p * B + q] = Z[s][static_cast<size_t>(i * B + p) * dimT + q];
+  if (!alg.combine(Y.ptr, 0.0, Vb.ptr, blk, m, coef)) return false;
+  if (!apply(false, Y.ptr, W.ptr)) return false;
+  std::vector<double> Th, Gr;
+  if (!alg.gram(Y.ptr, blk, 1, W.ptr, Th)) return false; // Y^T A Y
+  std::vector<double> negTh(Th.size());
+  for (size_t t = 0; t < Th.size(); ++t) negTh[t] = -Th[t];
+  if (!alg.combine(W.ptr, 1.0, Y.ptr, blk, 1, negTh)) return false; // A Y - Y (Y^T A Y)
+  if (!alg.gram(W.ptr, blk, 1, W.ptr, Gr)) return false;
+  res.evals.assign(static_cast<size_t>(ns) * o.nev, 0.0);
+  res.residuals.assign(static_cast<size_t>(ns) * o.nev, 0.0);
+  double rmax = 0.0;
+  for (int s = 0; s < ns; ++s)
+    for (int e = 0; e < o.nev; ++e) {
+      double lam = 0.0, r2 = 0.0;
+      for (int c = 0; c < multiplicity; ++c) {
+        int const q = multiplicity * e + c;
+        lam += Th[static_cast<size_t>(s) * V + q * B + q];
+        r2 = std::max(r2, Gr[static_cast<size_t>(s) * V + q * B + q]);
+      }
+      lam /= multiplicity;
+      double const rel = std::sqrt(std::max(0.0, r2)) / std::max(1.0, std::fabs(lam));
+      res.evals[static_cast<size_t>(s) * o.nev + e] = lam;
+      res.residuals[static_cast<size_t>(s) * o.nev + e] = rel;
+      rmax = std::max(rmax, rel);
+    }
+  res.first_rayleigh = Th[0];
+  res.first_residual = std::sqrt(std::max(0.0, Gr[0]));
+  if (d_ritz_block != nullptr)
+    LSAMD_HIP(hipMemcpyAsync(d_ritz_block, Y.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  res.cycles = cycles_run;
+  res.converged = done;
+  res.fp64_cycles = fp64_cycles;
+  res.blocks = blocks_done;
+  res.products = prods;
+  res.product_seconds = prod_s;
+  res.max_residual = rmax;
+  res.total_seconds = seconds_since(t_start);
+  return true;
+}
+
+} // namespace lsamd

@@ -118,6 +118,16 @@ __global__ __launch_bounds__(kBlock) void f32_to_f64_kernel(float const *__restr
   if (i < n) y[i] = static_cast<double>(x[i]);
 }
 
+// out = i v for a complex vector stored as interleaved (re, im) pairs.
+__global__ __launch_bounds__(kBlock) void times_i_kernel(double const *__restrict__ v, double *__restrict__ out,
+                                                         int64_t pairs) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= pairs) return;
+  double const re = v[2 * i], im = v[2 * i + 1];
+  out[2 * i] = -im;
+  out[2 * i + 1] = re;
+}
+
 // Seeded N(0, 1) entries: splitmix64 of (seed, i) and Box-Muller.
 __device__ __forceinline__ uint64_t splitmix(uint64_t z) {
   z += 0x9e3779b97f4a7c15ull;
@@ -134,36 +144,6 @@ __global__ __launch_bounds__(kBlock) void random_normal_kernel(uint64_t seed, do
   double const u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
   double const u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
   x[i] = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
-}
-
-// Block layout conversions: V columns (n x b, column-major) <-> row-major
-// block vectors [n][b] used by the block SpMV.
-__global__ __launch_bounds__(kBlock) void cols_to_rows_f32_kernel(double const *__restrict__ V, int64_t ld, int b,
-                                                                  float *__restrict__ X, int64_t n) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  for (int q = 0; q < b; ++q) X[i * b + q] = static_cast<float>(V[static_cast<int64_t>(q) * ld + i]);
-}
-
-__global__ __launch_bounds__(kBlock) void rows_to_cols_f64_kernel(float const *__restrict__ X, int b,
-                                                                  double *__restrict__ W, int64_t ld, int64_t n) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  for (int q = 0; q < b; ++q) W[static_cast<int64_t>(q) * ld + i] = static_cast<double>(X[i * b + q]);
-}
-
-__global__ __launch_bounds__(kBlock) void cols_to_rows_f64_kernel(double const *__restrict__ V, int64_t ld, int b,
-                                                                  double *__restrict__ X, int64_t n) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  for (int q = 0; q < b; ++q) X[i * b + q] = V[static_cast<int64_t>(q) * ld + i];
-}
-
-__global__ __launch_bounds__(kBlock) void rows_to_cols_f64d_kernel(double const *__restrict__ X, int b,
-                                                                   double *__restrict__ W, int64_t ld, int64_t n) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  for (int q = 0; q < b; ++q) W[static_cast<int64_t>(q) * ld + i] = X[i * b + q];
 }
 
 // ---------------------------------------------------------------------------
@@ -300,9 +280,14 @@ bool lanczos_run(OperatorCore &op, int mode, int max_steps, int k, double tol, u
                  double &matvec_seconds, int64_t &matvec_calls) {
   int64_t const n = vector_length(op, mode);
   if (n <= 0) return fail("empty basis");
-  int const m_max = static_cast<int>(std::min<int64_t>(max_steps, n));
-  DeviceArray<double> V, w, h;
+  // Complex vectors (mode 3): the Krylov vectors are kept orthogonal to i V
+  // as well, which is Gram-Schmidt in the complex inner product; without it
+  // the real embedding's doubled eigenvalues reappear as ghost copies.
+  bool const cplx = mode == 3;
+  int const m_max = static_cast<int>(std::min<int64_t>(max_steps, cplx ? n / 2 : n));
+  DeviceArray<double> V, w, h, JV;
   if (!V.allocate(static_cast<size_t>(n) * (m_max + 1)) || !w.allocate(n) || !h.allocate(m_max + 8)) return false;
+  if (cplx && !JV.allocate(static_cast<size_t>(n) * (m_max + 1))) return false;
   Blas blas;
   if (!blas.init(n, m_max + 1)) return false;
   Product prod;
@@ -316,6 +301,7 @@ bool lanczos_run(OperatorCore &op, int mode, int max_steps, int k, double tol, u
   double nrm2 = 0;
   if (!blas.dot(V.ptr, V.ptr, nrm2)) return false;
   scale_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, 1.0 / std::sqrt(nrm2), V.ptr, n);
+  if (cplx) times_i_kernel<<<grid_for(n / 2, kBlock), kBlock, 0, stream()>>>(V.ptr, JV.ptr, n / 2);
   std::vector<double> alpha, beta;
   std::vector<double> hh(m_max + 8);
   std::vector<double> ev, Z;
@@ -332,6 +318,10 @@ bool lanczos_run(OperatorCore &op, int mode, int max_steps, int k, double tol, u
     for (int pass = 0; pass < 2; ++pass) {
       if (!blas.multi_dot(V.ptr, n, j + 1, w.ptr, h.ptr)) return false;
       gemv_sub_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, n, j + 1, h.ptr, w.ptr, n);
+      if (cplx) {
+        if (!blas.multi_dot(JV.ptr, n, j + 1, w.ptr, h.ptr)) return false;
+        gemv_sub_kernel<<<g, kBlock, 0, stream()>>>(JV.ptr, n, j + 1, h.ptr, w.ptr, n);
+      }
     }
     double bb = 0;
     if (!blas.dot(w.ptr, w.ptr, bb)) return false;
@@ -355,6 +345,10 @@ bool lanczos_run(OperatorCore &op, int mode, int max_steps, int k, double tol, u
     if (stop) break;
     beta.push_back(b);
     scale_kernel<<<g, kBlock, 0, stream()>>>(w.ptr, 1.0 / b, V.ptr + static_cast<int64_t>(j + 1) * n, n);
+    if (cplx)
+      times_i_kernel<<<grid_for(n / 2, kBlock), kBlock, 0, stream()>>>(V.ptr + static_cast<int64_t>(j + 1) * n,
+                                                                       JV.ptr + static_cast<int64_t>(j + 1) * n,
+                                                                       n / 2);
   }
   std::vector<double> T(static_cast<size_t>(m) * m, 0.0);
   for (int i = 0; i < m; ++i) {
@@ -396,156 +390,6 @@ bool rayleigh_residual(OperatorCore &op, double const *d_u, double &theta, doubl
   double rr = 0;
   if (!blas.dot(w.ptr, w.ptr, rr)) return false;
   resid = std::sqrt(rr / uu);
-  return true;
-}
-
-// Block Lanczos with block size b (<= 16) and the FP32 (mode 1) or FP64
-// (mode 0) block CSR product; FP64 vectors and orthogonalisation.
-bool block_lanczos_run(OperatorCore &op, int mode, int b, int max_blocks, int k, double tol, uint64_t seed,
-                       double *d_ritz, std::vector<double> &evals, int &blocks_done, double &matvec_seconds,
-                       int64_t &matvec_calls) {
-  int64_t const n = op.number_states;
-  if (b < 1 || b > 16) return fail("block size must be in [1, 16]");
-  int const nb_max = static_cast<int>(std::min<int64_t>(max_blocks, std::max<int64_t>(1, n / b - 1)));
-  int const cols = (nb_max + 1) * b;
-  DeviceArray<double> V, W, Hd;
-  DeviceArray<float> Xf, Yf;
-  DeviceArray<double> Xd, Yd;
-  if (!V.allocate(static_cast<size_t>(n) * cols) || !W.allocate(static_cast<size_t>(n) * b) ||
-      !Hd.allocate(static_cast<size_t>(cols) * b + 256))
-    return false;
-  if (mode == 1) {
-    if (!Xf.allocate(static_cast<size_t>(n) * b) || !Yf.allocate(static_cast<size_t>(n) * b)) return false;
-  } else {
-    if (!Xd.allocate(static_cast<size_t>(n) * b) || !Yd.allocate(static_cast<size_t>(n) * b)) return false;
-  }
-  Blas blas;
-  if (!blas.init(n, cols + 16)) return false;
-  int const g = grid_for(n, kBlock);
-  double mv_seconds = 0;
-  int64_t mv_calls = 0;
-  // Orthonormalise the b columns at P in place (modified Gram-Schmidt,
-  // FP64); returns the b x b upper-triangular R (row-major) on the host.
-  auto orthonormalise = [&](double *P, std::vector<double> &R) -> bool {
-    R.assign(static_cast<size_t>(b) * b, 0.0);
-    for (int q = 0; q < b; ++q) {
-      double *pq = P + static_cast<int64_t>(q) * n;
-      for (int pass = 0; pass < 2; ++pass)
-        for (int p = 0; p < q; ++p) {
-          double d = 0;
-          if (!blas.dot(P + static_cast<int64_t>(p) * n, pq, d)) return false;
-          R[static_cast<size_t>(p) * b + q] += d;
-          axpy_kernel<<<g, kBlock, 0, stream()>>>(-d, P + static_cast<int64_t>(p) * n, pq, n);
-        }
-      double nn = 0;
-      if (!blas.dot(pq, pq, nn)) return false;
-      double const r = std::sqrt(nn);
-      R[static_cast<size_t>(q) * b + q] = r;
-      if (r > 0) scale_kernel<<<g, kBlock, 0, stream()>>>(pq, 1.0 / r, pq, n);
-    }
-    return true;
-  };
-  for (int q = 0; q < b; ++q)
-    random_normal_kernel<<<g, kBlock, 0, stream()>>>(seed + 7919ull * q, V.ptr + static_cast<int64_t>(q) * n, n);
-  std::vector<double> R;
-  if (!orthonormalise(V.ptr, R)) return false;
-  // Projected matrix T (block tridiagonal), kept dense on the host.
-  int const tdim = cols;
-  std::vector<double> T(static_cast<size_t>(tdim) * tdim, 0.0);
-  std::vector<double> hbuf(static_cast<size_t>(cols) * b);
-  std::vector<double> ev, Z;
-  int nb = 0;
-  for (int j = 0; j < nb_max; ++j) {
-    double *Vj = V.ptr + static_cast<int64_t>(j) * b * n;
-    auto t0 = Clock::now();
-    if (mode == 1) {
-      cols_to_rows_f32_kernel<<<g, kBlock, 0, stream()>>>(Vj, n, b, Xf.ptr, n);
-      if (!operator_csr_f32_block(op, b, Xf.ptr, Yf.ptr)) return false;
-      rows_to_cols_f64_kernel<<<g, kBlock, 0, stream()>>>(Yf.ptr, b, W.ptr, n, n);
-    } else {
-      cols_to_rows_f64_kernel<<<g, kBlock, 0, stream()>>>(Vj, n, b, Xd.ptr, n);
-      if (!operator_csr_f64_block(op, b, Xd.ptr, Yd.ptr)) return false;
-      rows_to_cols_f64d_kernel<<<g, kBlock, 0, stream()>>>(Yd.ptr, b, W.ptr, n, n);
-    }
-    LSAMD_HIP(hipStreamSynchronize(stream()));
-    mv_seconds += seconds_since(t0);
-    mv_calls += 1;
-    // Full re-orthogonalisation of every column of W against V[0 .. (j+1) b),
-    // twice; the coefficients of the first pass against block j give A_j.
-    int const mcols = (j + 1) * b;
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int q = 0; q < b; ++q) {
-        double *wq = W.ptr + static_cast<int64_t>(q) * n;
-        if (!blas.multi_dot(V.ptr, n, mcols, wq, Hd.ptr)) return false;
-        LSAMD_HIP(hipMemcpyAsync(hbuf.data() + static_cast<size_t>(q) * cols, Hd.ptr, sizeof(double) * mcols,
-                                 hipMemcpyDeviceToHost, stream()));
-        gemv_sub_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, n, mcols, Hd.ptr, wq, n);
-      }
-      LSAMD_HIP(hipStreamSynchronize(stream()));
-      if (pass == 0) {
-        // A_j = V_j^T H V_j (symmetrised), B_{j-1} from the previous block
-        for (int p = 0; p < b; ++p)
-          for (int q = 0; q < b; ++q) {
-            double const apq = hbuf[static_cast<size_t>(q) * cols + j * b + p];
-            double const aqp = hbuf[static_cast<size_t>(p) * cols + j * b + q];
-            T[static_cast<size_t>(j * b + p) * tdim + j * b + q] = 0.5 * (apq + aqp);
-          }
-      }
-    }
-    nb = j + 1;
-    if (!orthonormalise(W.ptr, R)) return false;
-    // T[(j+1)b + p, j b + q] = R[p][q]
-    if (j + 1 < nb_max + 1)
-      for (int p = 0; p < b; ++p)
-        for (int q = 0; q < b; ++q) {
-          double const r = R[static_cast<size_t>(p) * b + q];
-          T[static_cast<size_t>((j + 1) * b + p) * tdim + j * b + q] = r;
-          T[static_cast<size_t>(j * b + q) * tdim + (j + 1) * b + p] = r;
-        }
-    bool stop = (j + 1 == nb_max);
-    if (!stop && (nb % 4 == 0 || nb * b >= 200)) {
-      int const m = nb * b;
-      std::vector<double> Tm(static_cast<size_t>(m) * m);
-      for (int r = 0; r < m; ++r)
-        for (int c = 0; c < m; ++c) Tm[static_cast<size_t>(r) * m + c] = T[static_cast<size_t>(r) * tdim + c];
-      jacobi_eigh(Tm, m, ev, Z);
-      double worst = 0;
-      for (int q = 0; q < std::min(k, m); ++q) {
-        // residual bound ||R_j Z_last,q||
-        double rs = 0;
-        for (int p = 0; p < b; ++p) {
-          double acc = 0;
-          for (int c = 0; c < b; ++c)
-            acc += R[static_cast<size_t>(p) * b + c] * Z[static_cast<size_t>((nb - 1) * b + c) * m + q];
-          rs += acc * acc;
-        }
-        worst = std::max(worst, std::sqrt(rs) / std::max(1.0, std::fabs(ev[q])));
-      }
-      if (worst < tol) stop = true;
-    }
-    if (stop) break;
-    LSAMD_HIP(hipMemcpyAsync(V.ptr + static_cast<int64_t>(j + 1) * b * n, W.ptr, sizeof(double) * n * b,
-                             hipMemcpyDeviceToDevice, stream()));
-  }
-  int const m = nb * b;
-  std::vector<double> Tm(static_cast<size_t>(m) * m);
-  for (int r = 0; r < m; ++r)
-    for (int c = 0; c < m; ++c) Tm[static_cast<size_t>(r) * m + c] = T[static_cast<size_t>(r) * tdim + c];
-  jacobi_eigh(Tm, m, ev, Z);
-  evals.assign(ev.begin(), ev.begin() + std::min(k, m));
-  blocks_done = nb;
-  if (d_ritz != nullptr) {
-    std::vector<double> s(m);
-    for (int i = 0; i < m; ++i) s[i] = Z[static_cast<size_t>(i) * m + 0];
-    LSAMD_HIP(hipMemcpyAsync(Hd.ptr, s.data(), sizeof(double) * m, hipMemcpyHostToDevice, stream()));
-    gemv_kernel<<<g, kBlock, 0, stream()>>>(V.ptr, n, m, Hd.ptr, d_ritz, n);
-    double rr = 0;
-    if (!blas.dot(d_ritz, d_ritz, rr)) return false;
-    scale_kernel<<<g, kBlock, 0, stream()>>>(d_ritz, 1.0 / std::sqrt(rr), d_ritz, n);
-  }
-  LSAMD_HIP(hipStreamSynchronize(stream()));
-  matvec_seconds = mv_seconds;
-  matvec_calls = mv_calls;
   return true;
 }
 
@@ -609,31 +453,107 @@ extern "C" int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int 
   return 0;
 }
 
+namespace {
+// Block Lanczos on one operator through the segmented solver (one segment).
+bool operator_block_solve(OperatorCore &core, int mode, BlockKrylovOptions const &o, BlockKrylovResult &r,
+                          double *d_ritz_block) {
+  int64_t const n = core.number_states;
+  int64_t const count = n * o.block;
+  DeviceArray<float> X32;
+  if (mode == 1 && !X32.allocate(static_cast<size_t>(std::max<int64_t>(count, 1)))) return false;
+  BlockProduct prod = [&](bool fp32, double const *x, double *y) -> bool {
+    if (fp32) {
+      if (!block_to_f32(x, X32.ptr, count)) return false;
+      return operator_csr_f32_block_out64(core, o.block, X32.ptr, y);
+    }
+    return operator_csr_f64_block(core, o.block, x, y);
+  };
+  std::vector<int64_t> const seg{0, n};
+  return block_krylov_solve(seg, 1, prod, o, r, d_ritz_block);
+}
+
+void fill_info(ls_amd_eig_info *info, BlockKrylovResult const &r) {
+  if (info == nullptr) return;
+  info->cycles = r.cycles;
+  info->fp64_cycles = r.fp64_cycles;
+  info->blocks = r.blocks;
+  info->converged = r.converged ? 1 : 0;
+  info->products = r.products;
+  info->product_seconds = r.product_seconds;
+  info->total_seconds = r.total_seconds;
+  info->max_residual = r.max_residual;
+}
+} // namespace
+
+extern "C" int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size, int nev, int max_blocks,
+                                       int keep_blocks, int max_cycles, double tol, double switch_tol, uint64_t seed,
+                                       double *evals, double *residuals, double *d_ritz_out, ls_amd_eig_info *info) {
+  if (op == nullptr || evals == nullptr) return fail("null argument") ? 0 : -1;
+  if (mode != 0 && mode != 1) return fail("mode must be 0 (FP64 block CSR) or 1 (FP32 then FP64)") ? 0 : -1;
+  Csr const &c = op->core.csr;
+  if (c.is_complex) return fail("the CSR holds complex values: use a sector batch") ? 0 : -1;
+  if (!c.has64 || (mode == 1 && !c.has32))
+    return fail("build the CSR cache in both precisions first (precision 96)") ? 0 : -1;
+  BlockKrylovOptions o;
+  o.block = block_size;
+  o.nev = nev;
+  o.max_blocks = max_blocks;
+  o.keep = keep_blocks;
+  o.max_cycles = max_cycles;
+  o.tol = tol;
+  o.switch_tol = switch_tol;
+  o.seed = seed;
+  o.fp32 = mode == 1;
+  int64_t const n = op->core.number_states;
+  DeviceArray<double> Yb;
+  if (d_ritz_out != nullptr && !Yb.allocate(static_cast<size_t>(n) * block_size)) return -1;
+  BlockKrylovResult r;
+  if (!operator_block_solve(op->core, mode, o, r, d_ritz_out != nullptr ? Yb.ptr : nullptr)) return -1;
+  std::copy(r.evals.begin(), r.evals.end(), evals);
+  if (residuals != nullptr) std::copy(r.residuals.begin(), r.residuals.end(), residuals);
+  if (d_ritz_out != nullptr) {
+    if (!block_column(Yb.ptr, block_size, 0, d_ritz_out, n)) return -1;
+    if (hipStreamSynchronize(stream()) != hipSuccess) return fail("stream synchronisation failed") ? 0 : -1;
+  }
+  fill_info(info, r);
+  return 0;
+}
+
 extern "C" int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_size, int max_blocks, int k, double tol,
                                     uint64_t seed, int refine_steps, double *d_ritz_out, ls_amd_lanczos_result *out) {
   if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
   if (mode != 0 && mode != 1) return fail("mode must be 0 (FP64 block CSR) or 1 (FP32 block CSR)") ? 0 : -1;
+  if (op->core.csr.is_complex) return fail("the CSR holds complex values: use a sector batch") ? 0 : -1;
   if ((mode == 0 && !op->core.csr.has64) || (mode == 1 && !op->core.csr.has32))
     return fail("build the CSR cache in the requested precision first") ? 0 : -1;
+  if (!op->core.csr.has64) return fail("the FP64 CSR is needed for the final FP64 cycles (precision 96)") ? 0 : -1;
   auto t0 = Clock::now();
   int64_t const n = op->core.number_states;
-  DeviceArray<double> u;
-  if (!u.allocate(static_cast<size_t>(n))) return -1;
-  std::vector<double> evals;
-  int blocks = 0;
-  double mv = 0;
-  int64_t calls = 0;
-  if (!block_lanczos_run(op->core, mode, block_size, max_blocks, std::max(1, std::min(k, 8)), tol, seed, u.ptr, evals,
-                         blocks, mv, calls))
-    return -1;
+  BlockKrylovOptions o;
+  o.block = block_size;
+  o.nev = std::max(1, std::min(k, block_size));
+  o.max_blocks = std::max(4, max_blocks);
+  o.keep = 2;
+  o.max_cycles = 60;
+  o.tol = tol;
+  o.switch_tol = std::max(tol, 1e-5);
+  o.seed = seed;
+  o.fp32 = mode == 1;
+  DeviceArray<double> Yb, u;
+  if (!Yb.allocate(static_cast<size_t>(n) * block_size) || !u.allocate(static_cast<size_t>(n))) return -1;
+  BlockKrylovResult r;
+  if (!operator_block_solve(op->core, mode, o, r, Yb.ptr)) return -1;
+  if (!block_column(Yb.ptr, block_size, 0, u.ptr, n)) return -1;
+  std::vector<double> evals = r.evals;
+  double mv = r.product_seconds;
+  int64_t calls = r.products;
   out->refine_steps = 0;
   if (refine_steps > 0) {
     std::vector<double> ev2;
     int st2 = 0;
     double mv2 = 0;
     int64_t c2 = 0;
-    int const form = op->core.csr.has64 ? 0 : 2;
-    if (!lanczos_run(op->core, form, refine_steps, 1, 0.0, seed, u.ptr, u.ptr, ev2, st2, mv2, c2)) return -1;
+    if (!lanczos_run(op->core, 0, refine_steps, 1, 0.0, seed, u.ptr, u.ptr, ev2, st2, mv2, c2)) return -1;
     evals[0] = ev2[0];
     mv += mv2;
     calls += c2;
@@ -641,7 +561,7 @@ extern "C" int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_siz
   }
   double theta = 0, resid = 0;
   if (!rayleigh_residual(op->core, u.ptr, theta, resid, false)) return -1;
-  fill_result(out, evals, blocks, mv, calls, seconds_since(t0));
+  fill_result(out, evals, r.blocks, mv, calls, seconds_since(t0));
   out->rayleigh = theta;
   out->residual_norm = resid;
   if (d_ritz_out != nullptr)

@@ -622,13 +622,13 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
   }
 }
 
-template <int K, int LPR, typename T>
+template <int K, int LPR, typename T, typename TO>
 __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                         uint32_t const *__restrict__ cols,
                                                         T const *__restrict__ vals,
                                                         T const *__restrict__ diag,
                                                         T const *__restrict__ x,
-                                                        T *__restrict__ y) {
+                                                        TO *__restrict__ y) {
   int64_t const gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t const row = gid / LPR;
   int const sub = static_cast<int>(gid % LPR);
@@ -652,7 +652,7 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const
     T const d = diag[row];
 #pragma unroll
     for (int q = 0; q < K; ++q)
-      y[static_cast<size_t>(row) * K + q] = d * x[static_cast<size_t>(row) * K + q] + acc[q];
+      y[static_cast<size_t>(row) * K + q] = static_cast<TO>(d * x[static_cast<size_t>(row) * K + q] + acc[q]);
   }
 }
 
@@ -707,7 +707,26 @@ bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_
   timer_begin(kTimerCsrF32);
   bool ok = dispatch_k(k, [&]<int K>() -> bool {
     return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
-      csr_block_kernel<K, LPR, float><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
+      csr_block_kernel<K, LPR, float, float><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
+          n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    });
+  });
+  timer_end(kTimerCsrF32);
+  return ok;
+}
+
+bool operator_csr_f32_block_out64(OperatorCore &op, int k, float const *d_x, double *d_y) {
+  Csr const &c = op.csr;
+  if (c.is_complex) return fail("the CSR holds complex values: use a sector batch for its products");
+  if (!c.has32) return fail("FP32 CSR has not been built");
+  int64_t const n = c.rows;
+  if (n == 0) return true;
+  timer_begin(kTimerCsrF32);
+  bool ok = dispatch_k(k, [&]<int K>() -> bool {
+    return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
+      csr_block_kernel<K, LPR, float, double><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
           n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
       LSAMD_HIP(hipGetLastError());
       return true;
@@ -726,7 +745,7 @@ bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *
   timer_begin(kTimerCsrF64Block);
   bool ok = dispatch_k(k, [&]<int K>() -> bool {
     return dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
-      csr_block_kernel<K, LPR, double><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
+      csr_block_kernel<K, LPR, double, double><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
           n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, d_x, d_y);
       LSAMD_HIP(hipGetLastError());
       return true;

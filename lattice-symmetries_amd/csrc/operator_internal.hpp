@@ -1,6 +1,8 @@
 // operator_internal.hpp -- operator objects shared by operator.hip and abi.cpp.
 #pragma once
 
+#include <functional>
+
 #include "lsamd_internal.hpp"
 
 namespace lsamd {
@@ -76,6 +78,45 @@ bool apply_off_diag_device(Terms const &t, int64_t count, uint64_t const *d_alph
                            int64_t &total);
 
 bool exclusive_scan(uint32_t const *d_counts, int64_t n, uint64_t *d_offsets, uint64_t &total);
+
+// FP32 block of k right-hand sides in, FP64 out (row-major [n][k]).
+bool operator_csr_f32_block_out64(OperatorCore &op, int k, float const *d_x, double *d_y);
+
+// Thick-restart block Lanczos on FP64 row-major [rows][block] blocks over
+// independent segments of rows (block_krylov.hip).
+struct BlockKrylovOptions {
+  int block = 8;       // columns per block
+  int nev = 4;         // eigenvalues wanted per segment
+  int max_blocks = 12; // Krylov blocks per cycle
+  int keep = 2;        // Ritz blocks kept at a restart
+  int max_cycles = 40;
+  double tol = 1e-7;        // relative residual for convergence with the FP64 product
+  double switch_tol = 1e-5; // relative residual at which FP32 gives way to FP64
+  uint64_t seed = 1;
+  bool fp32 = true;         // start with the FP32 product
+};
+
+struct BlockKrylovResult {
+  std::vector<double> evals;     // segments * nev, ascending per segment
+  std::vector<double> residuals; // FP64 relative residual norms
+  std::vector<double> history;   // largest residual bound after every cycle
+  int cycles = 0, fp64_cycles = 0, blocks = 0;
+  bool converged = false;
+  int64_t products = 0;
+  double product_seconds = 0.0, total_seconds = 0.0, max_residual = 0.0;
+  double first_rayleigh = 0.0, first_residual = 0.0; // lowest Ritz vector of segment 0
+};
+
+// y = A x on a block; fp32 selects the FP32 product (FP64 x and y).
+using BlockProduct = std::function<bool(bool fp32, double const *x, double *y)>;
+
+// Each eigenvalue appears `multiplicity` times in the segments' spectra (2 for
+// the real embedding of complex sectors); d_ritz_block (may be null) receives
+// the lowest Ritz block.
+bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, BlockProduct const &product,
+                        BlockKrylovOptions const &o, BlockKrylovResult &res, double *d_ritz_block);
+bool block_to_f32(double const *x, float *y, int64_t count);
+bool block_column(double const *X, int b, int col, double *out, int64_t rows);
 
 } // namespace lsamd
 

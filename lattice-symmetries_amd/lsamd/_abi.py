@@ -129,7 +129,7 @@ class ls_amd_lanczos_result(C.Structure):
     ]
 
 
-class ls_amd_sector_eig_info(C.Structure):
+class ls_amd_eig_info(C.Structure):
     _fields_ = [
         ("cycles", C.c_int),
         ("fp64_cycles", C.c_int),
@@ -236,6 +236,17 @@ SIGNATURES = {
                                  C.POINTER(ls_amd_lanczos_result)]),
     "ls_amd_block_lanczos": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int,
                                        _P, C.POINTER(ls_amd_lanczos_result)]),
+    "ls_amd_model_from_yaml": (_P, [C.c_char_p]),
+    "ls_amd_model_destroy": (None, [_P]),
+    "ls_amd_model_basis_info": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "ls_amd_model_symmetry": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ls_amd_model_number_terms": (C.c_int64, [_P, C.c_int, C.c_int]),
+    "ls_amd_model_terms": (C.c_int, [_P, C.c_int, C.c_int, _f64p, _u64p, _u64p, _u64p, _u64p, _u64p]),
+    "ls_amd_model_create_basis": (_P, [_P]),
+    "ls_amd_model_create_operator": (_P, [_P, C.c_int, _P]),
+    "ls_amd_block_eigensolve": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                          C.c_double, C.c_uint64, _f64p, _f64p, _P,
+                                          C.POINTER(ls_amd_eig_info)]),
     "ls_amd_sector_batch_create": (_P, [C.POINTER(_P), C.c_int]),
     "ls_amd_sector_batch_destroy": (None, [_P]),
     "ls_amd_sector_batch_count": (C.c_int, [_P]),
@@ -250,7 +261,7 @@ SIGNATURES = {
     "ls_amd_sector_batch_matvec_f64": (C.c_int, [_P, C.c_int, _P, _P]),
     "ls_amd_sector_batch_eigensolve": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                                  C.c_double, C.c_uint64, _f64p, _f64p,
-                                                 C.POINTER(ls_amd_sector_eig_info)]),
+                                                 C.POINTER(ls_amd_eig_info)]),
 }
 
 

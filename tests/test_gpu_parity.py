@@ -256,8 +256,16 @@ def test_lanczos_precision_forms(engine):
     assert r64["residual_norm"] < 1e-6
     r32 = dop.lanczos("fp32", max_steps=300, k=1, tol=1e-8, refine_steps=30)
     assert abs(r32["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
-    rb = dop.block_lanczos("fp32", block_size=8, max_blocks=60, k=4, tol=1e-7, refine_steps=30)
+    rb = dop.block_lanczos("fp32", block_size=8, max_blocks=12, k=4, tol=1e-7, refine_steps=30)
     assert abs(rb["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
+    # thick-restart block Lanczos: FP32 block product, then FP64 cycles
+    for mode in ("fp32", "fp64"):
+        be = dop.block_eigensolve(mode, block_size=8, nev=4, max_blocks=10, keep_blocks=2, max_cycles=40, tol=1e-8)
+        assert be["converged"], be
+        assert np.abs(be["eigenvalues"] - exact[:4]).max() <= 1e-10 * abs(exact[0]), (mode, be["eigenvalues"])
+        assert be["max_residual"] < 1e-7
+        if mode == "fp32":
+            assert be["fp64_cycles"] >= 1 and be["cycles"] > be["fp64_cycles"]
     # the next three Ritz values of the FP32 block run, at FP32 accuracy
     assert np.allclose(rb["eigenvalues"][1:4], exact[1:4], rtol=1e-5)
     rotf = dop.lanczos("otf", max_steps=300, k=1, tol=1e-13)
