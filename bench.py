@@ -34,6 +34,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy of known size), as recorded in
+    profiles/traffic_per_launch.json; None when this kernel was not profiled."""
+    try:
+        with open(os.path.join(REPO, "profiles", "traffic_per_launch.json")) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    e = table.get(workload, {}).get(kernel)
+    return (e["bytes"], e["source"]) if e else (None, None)
+
+
 def workload(L=6, J2=0.5):
     import lsamd
     from lsamd import models
@@ -111,6 +124,9 @@ def sectors_bench(args):
     nnz = nnz_off + dim
     bytes_per_launch = 76.0 * nnz_off + 140.0 * dim
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9
+    sector_kernel = (f"sector_spmv_kernel<8, {sched['lanes']}, float, float>"
+                     f"{' die-aware' if sched['die_aware'] else ''} x{sched['blocks']}")
+    traffic, traffic_src = profiled_traffic(f"chain{n}_sectors", sector_kernel)
     value = 8.0 * nnz * args.steps / t_steps
     del dx, dy
     r = batch.eigensolve(nev=4, block_size=8, max_blocks=args.sector_blocks, keep_blocks=2,
@@ -146,8 +162,8 @@ def sectors_bench(args):
                                "(k x spin inversion), block of 8 complex RHS, one graph-replayed launch",
                    "states": dim, "off_diagonal_nonzeros": nnz_off, "schedule": sched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sector_spmv_kernel",
-                     "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": sector_kernel, "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch},
         "setup_seconds": t_setup,
         "eigensolve": {"cycles": r["cycles"], "fp64_cycles": r["fp64_cycles"], "products": r["products"],
                        "product_seconds": r["product_seconds"], "seconds": r["total_seconds"],
@@ -158,6 +174,21 @@ def sectors_bench(args):
         "fp64_reference": checks,
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration(args):
+    """A plain device copy of known size (torch's copy kernel), run under the
+    profiler's FETCH_SIZE / WRITE_SIZE counters to scale them to bytes."""
+    import torch
+    nbytes = args.copy_mib << 20
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy-calibration", "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+                      "copies": args.steps}), flush=True)
 
 
 def main():
@@ -171,7 +202,8 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--block-cycle", type=int, default=12)
-    ap.add_argument("--workload", choices=["square6", "chain-sectors"], default="square6")
+    ap.add_argument("--workload", choices=["square6", "chain-sectors", "copy-calibration"], default="square6")
+    ap.add_argument("--copy-mib", type=int, default=2048)
     ap.add_argument("--chain", type=int, default=28)
     ap.add_argument("--sector-blocks", type=int, default=12)
     ap.add_argument("--sector-cycles", type=int, default=40)
@@ -183,6 +215,9 @@ def main():
               "is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "copy-calibration":
+        copy_calibration(args)
+        return
     import lsamd
     from lsamd import _abi
     lib = lsamd.load()
@@ -209,7 +244,10 @@ def main():
     nnz = nnz_off + dim  # stored off-diagonal entries + the diagonal
     log(f"csr: {nnz_off} off-diagonal nonzeros in {t_csr:.2f} s")
     lanes = op.autotune()
-    log(f"autotune: {lanes} lanes per row (fp64)")
+    sched = op.schedule
+    log(f"autotune: {sched}")
+    nt, die = sched["flags_f64"] & 1, (sched["flags_f64"] >> 1) & 1
+    kernel_name = f"csr_f64_kernel<{lanes}, {'true' if nt else 'false'}, {'true' if die else 'false'}>"
 
     rng = np.random.default_rng(42)
     x = lsamd.DeviceArray.from_numpy(rng.standard_normal(dim))
@@ -231,6 +269,7 @@ def main():
     k_ms = float(np.mean(kernel_ms))
     bytes_per_launch = 20.0 * nnz_off + 32.0 * dim
     achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+    traffic, traffic_src = profiled_traffic(f"square{args.lattice}", kernel_name)
 
     # ---- FP32 block of 8 ----
     X8 = lsamd.DeviceArray.from_numpy(rng.standard_normal((dim, 8)).astype(np.float32))
@@ -290,10 +329,10 @@ def main():
         "config": {"workload": f"J1-J2 {args.lattice}x{args.lattice} square lattice, J2=0.5, Hamming weight {h}, "
                                "T x C4v x Z2 (576 elements), FP64 CSR H x",
                    "states": dim, "nonzeros": nnz, "off_diagonal_nonzeros": nnz_off,
-                   "lanes_per_row": lanes},
+                   "schedule": sched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "csr_f64_kernel", "kernel_ms": k_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": kernel_name, "kernel_ms": k_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "basis_build": {"states": dim, "candidates": candidates, "seconds": t_build,
                         "states_per_s": dim / t_build, "candidates_per_s": candidates / t_build},

@@ -300,9 +300,17 @@ int ls_amd_csr_matvec_f32(ls_amd_operator *op, int k, float const *d_x,
 /* FP64 block of k right-hand sides, row-major [n][k] (k <= 16). */
 int ls_amd_csr_matvec_f64_block(ls_amd_operator *op, int k, double const *d_x,
                                 double *d_y);
-/* Choose the CSR kernel schedule (lanes per row) by timing the candidates
-   on the current matrix; returns the chosen lanes per row. */
+/* Choose the CSR kernel schedules by timing the candidates on the current
+   matrix; returns the chosen FP64 lanes per row. */
 int ls_amd_operator_autotune(ls_amd_operator *op);
+/* Current CSR product schedule: out[5] = lanes per row (FP64), flags (FP64),
+   lanes per row (FP32 block), flags (FP32 block), persistent grid size.
+   flags: bit 0 non-temporal loads of the CSR stream, bit 1 die-aware
+   persistent grid (each accelerator die's workgroups walk one contiguous
+   range of rows). */
+int ls_amd_operator_schedule(ls_amd_operator const *op, int *out);
+int ls_amd_operator_set_schedule(ls_amd_operator *op, int lanes_f64,
+                                 int flags_f64, int lanes_f32, int flags_f32);
 
 /* Eigensolvers (the consumers of the product; the reference drives its
    product from scipy eigsh, python/lattice_symmetries/__init__.py:591-752,

@@ -764,3 +764,26 @@ extern "C" int ls_amd_csr_matvec_f64_block(ls_amd_operator *op, int k, double co
 }
 
 extern "C" int ls_amd_operator_autotune(ls_amd_operator *op) { return operator_autotune(op->core); }
+
+extern "C" int ls_amd_operator_schedule(ls_amd_operator const *op, int *out) {
+  if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
+  out[0] = op->core.lanes_f64;
+  out[1] = op->core.flags_f64;
+  out[2] = op->core.lanes_f32;
+  out[3] = op->core.flags_f32;
+  out[4] = op->core.persistent_blocks;
+  return 0;
+}
+
+extern "C" int ls_amd_operator_set_schedule(ls_amd_operator *op, int lanes_f64, int flags_f64, int lanes_f32,
+                                            int flags_f32) {
+  if (op == nullptr) return fail("null argument") ? 0 : -1;
+  for (int l : {lanes_f64, lanes_f32})
+    if (l < 1 || l > 64 || (l & (l - 1)) != 0) return fail("lanes per row must be a power of two <= 64") ? 0 : -1;
+  if (flags_f64 < 0 || flags_f64 > 3 || flags_f32 < 0 || flags_f32 > 3) return fail("schedule flags are 0..3") ? 0 : -1;
+  op->core.lanes_f64 = lanes_f64;
+  op->core.flags_f64 = flags_f64;
+  op->core.lanes_f32 = lanes_f32;
+  op->core.flags_f32 = flags_f32;
+  return 0;
+}

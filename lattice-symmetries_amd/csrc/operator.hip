@@ -598,7 +598,36 @@ bool operator_build_csr(OperatorCore &op, int precision) {
 // SpMV kernels: LPR lanes share one row
 // ---------------------------------------------------------------------------
 
-template <int LPR>
+// Stream loads of the CSR arrays, optionally non-temporal so that the
+// stream does not displace the gathered vector from L2.
+template <bool NT, typename T> __device__ __forceinline__ T ld_stream(T const *p) {
+  if constexpr (NT) {
+    return __builtin_nontemporal_load(p);
+  } else {
+    return *p;
+  }
+}
+
+// Row tiles of 256 / LPR rows.  Die-aware (DIE): the grid is a fixed multiple
+// of 8 workgroups; workgroups are dealt to the 8 accelerator dies round-robin,
+// and those of die d walk the contiguous tile range [tiles d / 8, tiles (d+1)
+// / 8), so rows that gather from nearby columns share one die's L2.
+// Otherwise one workgroup per tile.
+template <bool DIE> __device__ __forceinline__ void tile_range(int64_t tiles, int64_t &t, int64_t &t_end,
+                                                              int64_t &t_step) {
+  if constexpr (DIE) {
+    int const die = static_cast<int>(blockIdx.x) % 8;
+    t = tiles * die / 8 + static_cast<int64_t>(blockIdx.x) / 8;
+    t_end = tiles * (die + 1) / 8;
+    t_step = static_cast<int64_t>(gridDim.x) / 8;
+  } else {
+    t = blockIdx.x;
+    t_end = t < tiles ? t + 1 : t;
+    t_step = 1;
+  }
+}
+
+template <int LPR, bool NT, bool DIE>
 __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                       uint32_t const *__restrict__ cols,
                                                       double const *__restrict__ vals,
@@ -606,53 +635,63 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
                                                       double const *__restrict__ x,
                                                       double *__restrict__ y, int accumulate,
                                                       int has_diag) {
-  int64_t const gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t const row = gid / LPR;
-  int const sub = static_cast<int>(gid % LPR);
-  double acc = 0.0;
-  if (row < n) {
-    int64_t const b = rp[row], e = rp[row + 1];
-    for (int64_t k = b + sub; k < e; k += LPR) acc = fma(vals[k], x[cols[k]], acc);
-  }
+  constexpr int kRows = 256 / LPR;
+  int64_t const tiles = (n + kRows - 1) / kRows;
+  int64_t t, t_end, t_step;
+  tile_range<DIE>(tiles, t, t_end, t_step);
+  int const sub = static_cast<int>(threadIdx.x) % LPR;
+  for (; t < t_end; t += t_step) {
+    int64_t const row = t * kRows + static_cast<int>(threadIdx.x) / LPR;
+    double acc = 0.0;
+    if (row < n) {
+      int64_t const b = rp[row], e = rp[row + 1];
+      for (int64_t k = b + sub; k < e; k += LPR) acc = fma(ld_stream<NT>(vals + k), x[ld_stream<NT>(cols + k)], acc);
+    }
 #pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-  if (row < n && sub == 0) {
-    double const base = has_diag ? diag[row] * x[row] : (accumulate ? y[row] : 0.0);
-    y[row] = base + acc;
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+    if (row < n && sub == 0) {
+      double const base = has_diag ? diag[row] * x[row] : (accumulate ? y[row] : 0.0);
+      y[row] = base + acc;
+    }
   }
 }
 
-template <int K, int LPR, typename T, typename TO>
+template <int K, int LPR, typename T, typename TO, bool NT, bool DIE>
 __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                         uint32_t const *__restrict__ cols,
                                                         T const *__restrict__ vals,
                                                         T const *__restrict__ diag,
                                                         T const *__restrict__ x,
                                                         TO *__restrict__ y) {
-  int64_t const gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t const row = gid / LPR;
-  int const sub = static_cast<int>(gid % LPR);
-  T acc[K];
+  constexpr int kRows = 256 / LPR;
+  int64_t const tiles = (n + kRows - 1) / kRows;
+  int64_t t, t_end, t_step;
+  tile_range<DIE>(tiles, t, t_end, t_step);
+  int const sub = static_cast<int>(threadIdx.x) % LPR;
+  for (; t < t_end; t += t_step) {
+    int64_t const row = t * kRows + static_cast<int>(threadIdx.x) / LPR;
+    T acc[K];
 #pragma unroll
-  for (int q = 0; q < K; ++q) acc[q] = T(0);
-  if (row < n) {
-    int64_t const b = rp[row], e = rp[row + 1];
-    for (int64_t k = b + sub; k < e; k += LPR) {
-      T const v = vals[k];
-      T const *xr = x + static_cast<size_t>(cols[k]) * K;
+    for (int q = 0; q < K; ++q) acc[q] = T(0);
+    if (row < n) {
+      int64_t const b = rp[row], e = rp[row + 1];
+      for (int64_t k = b + sub; k < e; k += LPR) {
+        T const v = ld_stream<NT>(vals + k);
+        T const *xr = x + static_cast<size_t>(ld_stream<NT>(cols + k)) * K;
 #pragma unroll
-      for (int q = 0; q < K; ++q) acc[q] = fma(v, xr[q], acc[q]);
+        for (int q = 0; q < K; ++q) acc[q] = fma(v, xr[q], acc[q]);
+      }
     }
-  }
-#pragma unroll
-  for (int q = 0; q < K; ++q)
-#pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, LPR);
-  if (row < n && sub == 0) {
-    T const d = diag[row];
 #pragma unroll
     for (int q = 0; q < K; ++q)
-      y[static_cast<size_t>(row) * K + q] = static_cast<TO>(d * x[static_cast<size_t>(row) * K + q] + acc[q]);
+#pragma unroll
+      for (int off = LPR / 2; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, LPR);
+    if (row < n && sub == 0) {
+      T const d = diag[row];
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+        y[static_cast<size_t>(row) * K + q] = static_cast<TO>(d * x[static_cast<size_t>(row) * K + q] + acc[q]);
+    }
   }
 }
 
@@ -667,6 +706,22 @@ template <typename F> bool dispatch_lpr(int lpr, F &&f) {
   case 64: return f.template operator()<64>();
   default: return fail("lanes per row must be a power of two <= 64");
   }
+}
+
+// flags: bit 0 non-temporal stream loads, bit 1 die-aware persistent grid
+template <typename F> bool dispatch_flags(int flags, F &&f) {
+  switch (flags & 3) {
+  case 0: return f.template operator()<false, false>();
+  case 1: return f.template operator()<true, false>();
+  case 2: return f.template operator()<false, true>();
+  default: return f.template operator()<true, true>();
+  }
+}
+
+// Grid of a product launch: one workgroup per tile, or the persistent grid.
+int product_grid(OperatorCore const &op, int64_t n, int lpr, int flags) {
+  if (flags & 2) return op.persistent_blocks;
+  return grid_for(n * lpr, 256);
 }
 
 template <typename F> bool dispatch_k(int k, F &&f) {
@@ -687,12 +742,15 @@ bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool acc
   int64_t const n = c.rows;
   if (n == 0) return true;
   timer_begin(kTimerCsrF64);
+  int const grid = product_grid(op, n, op.lanes_f64, op.flags_f64);
   bool ok = dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
-    csr_f64_kernel<LPR><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
-        n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, d_x, d_y, accumulate ? 1 : 0,
-        op.diag.count > 0 ? 1 : 0);
-    LSAMD_HIP(hipGetLastError());
-    return true;
+    return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
+      csr_f64_kernel<LPR, NT, DIE><<<grid, 256, 0, stream()>>>(n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr,
+                                                               c.diag64.ptr, d_x, d_y, accumulate ? 1 : 0,
+                                                               op.diag.count > 0 ? 1 : 0);
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    });
   });
   timer_end(kTimerCsrF64);
   return ok;
@@ -707,10 +765,13 @@ bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_
   timer_begin(kTimerCsrF32);
   bool ok = dispatch_k(k, [&]<int K>() -> bool {
     return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
-      csr_block_kernel<K, LPR, float, float><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
-          n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
-      LSAMD_HIP(hipGetLastError());
-      return true;
+      int const grid = product_grid(op, n, LPR, op.flags_f32);
+      return dispatch_flags(op.flags_f32, [&]<bool NT, bool DIE>() -> bool {
+        csr_block_kernel<K, LPR, float, float, NT, DIE><<<grid, 256, 0, stream()>>>(
+            n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
+        LSAMD_HIP(hipGetLastError());
+        return true;
+      });
     });
   });
   timer_end(kTimerCsrF32);
@@ -726,10 +787,13 @@ bool operator_csr_f32_block_out64(OperatorCore &op, int k, float const *d_x, dou
   timer_begin(kTimerCsrF32);
   bool ok = dispatch_k(k, [&]<int K>() -> bool {
     return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
-      csr_block_kernel<K, LPR, float, double><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
-          n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
-      LSAMD_HIP(hipGetLastError());
-      return true;
+      int const grid = product_grid(op, n, LPR, op.flags_f32);
+      return dispatch_flags(op.flags_f32, [&]<bool NT, bool DIE>() -> bool {
+        csr_block_kernel<K, LPR, float, double, NT, DIE><<<grid, 256, 0, stream()>>>(
+            n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
+        LSAMD_HIP(hipGetLastError());
+        return true;
+      });
     });
   });
   timer_end(kTimerCsrF32);
@@ -745,22 +809,30 @@ bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *
   timer_begin(kTimerCsrF64Block);
   bool ok = dispatch_k(k, [&]<int K>() -> bool {
     return dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
-      csr_block_kernel<K, LPR, double, double><<<grid_for(n * LPR, 256), 256, 0, stream()>>>(
-          n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, d_x, d_y);
-      LSAMD_HIP(hipGetLastError());
-      return true;
+      int const grid = product_grid(op, n, LPR, op.flags_f64);
+      return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
+        csr_block_kernel<K, LPR, double, double, NT, DIE><<<grid, 256, 0, stream()>>>(
+            n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, d_x, d_y);
+        LSAMD_HIP(hipGetLastError());
+        return true;
+      });
     });
   });
   timer_end(kTimerCsrF64Block);
   return ok;
 }
 
-// Times every lanes-per-row schedule on the cached matrix and keeps the
-// fastest, separately for the FP64 kernel and the FP32 block-of-8 kernel.
+// Times every product schedule (lanes per row x stream-load kind x grid
+// mapping) on the cached matrix and keeps the fastest, separately for the
+// FP64 kernel and the FP32 block-of-8 kernel.
 int operator_autotune(OperatorCore &op) {
   Csr const &c = op.csr;
   int64_t const n = c.rows;
   if (n == 0 || c.is_complex) return op.lanes_f64;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    op.persistent_blocks = std::max(8, (prop.multiProcessorCount * 4 / 8) * 8);
   int const cand[] = {2, 4, 8, 16, 32, 64};
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
@@ -780,16 +852,20 @@ int operator_autotune(OperatorCore &op) {
     if (x.allocate(n) && y.allocate(n)) {
       (void)hipMemsetAsync(x.ptr, 0, sizeof(double) * n, stream());
       float best = 1e30f;
-      int best_l = op.lanes_f64;
-      for (int l : cand) {
-        op.lanes_f64 = l;
-        float const ms = time_it([&] { (void)operator_csr_f64(op, x.ptr, y.ptr, false); });
-        if (ms < best) {
-          best = ms;
-          best_l = l;
+      int best_l = op.lanes_f64, best_f = op.flags_f64;
+      for (int f = 0; f < 4; ++f)
+        for (int l : cand) {
+          op.lanes_f64 = l;
+          op.flags_f64 = f;
+          float const ms = time_it([&] { (void)operator_csr_f64(op, x.ptr, y.ptr, false); });
+          if (ms < best) {
+            best = ms;
+            best_l = l;
+            best_f = f;
+          }
         }
-      }
       op.lanes_f64 = best_l;
+      op.flags_f64 = best_f;
     }
   }
   if (c.has32) {
@@ -797,16 +873,20 @@ int operator_autotune(OperatorCore &op) {
     if (x.allocate(static_cast<size_t>(n) * 8) && y.allocate(static_cast<size_t>(n) * 8)) {
       (void)hipMemsetAsync(x.ptr, 0, sizeof(float) * n * 8, stream());
       float best = 1e30f;
-      int best_l = op.lanes_f32;
-      for (int l : cand) {
-        op.lanes_f32 = l;
-        float const ms = time_it([&] { (void)operator_csr_f32_block(op, 8, x.ptr, y.ptr); });
-        if (ms < best) {
-          best = ms;
-          best_l = l;
+      int best_l = op.lanes_f32, best_f = op.flags_f32;
+      for (int f = 0; f < 4; ++f)
+        for (int l : cand) {
+          op.lanes_f32 = l;
+          op.flags_f32 = f;
+          float const ms = time_it([&] { (void)operator_csr_f32_block(op, 8, x.ptr, y.ptr); });
+          if (ms < best) {
+            best = ms;
+            best_l = l;
+            best_f = f;
+          }
         }
-      }
       op.lanes_f32 = best_l;
+      op.flags_f32 = best_f;
     }
   }
   (void)hipEventDestroy(e0);

@@ -54,6 +54,9 @@ struct OperatorCore {
   Csr csr;
   int lanes_f64 = 8;  // lanes per row of the FP64 SpMV
   int lanes_f32 = 8;  // lanes per row of the FP32 block SpMV
+  int flags_f64 = 0;  // bit 0 non-temporal stream loads, bit 1 die-aware persistent grid
+  int flags_f32 = 0;
+  int persistent_blocks = 1024; // grid of the die-aware schedule (a multiple of 8)
   DeviceArray<int> d_err; // error flag + offending state
   DeviceArray<uint64_t> d_err_state;
 };

@@ -217,6 +217,43 @@ def test_matvec_matches_oracle(engine, name, n, h, gens, sectors, si, expr):
     assert np.abs(Y32 - ref8).max() <= 2e-5 * np.abs(ref8).max()
 
 
+def test_every_csr_schedule_gives_the_same_product(engine):
+    """Lanes per row x non-temporal stream loads x die-aware persistent grid:
+    all schedules of the FP64 and FP32 block products against the oracle."""
+    n, h = 20, 10
+    gens = [models.chain_translation(n), models.chain_parity(n)]
+    S = symmetries(gens, [0, 1])
+    expr = models.j1j2_chain(n, 1.0, 0.5)
+    og = O.Group(n, S.elements(), S.characters.real, S.characters.imag, -1)
+    db = lsamd.DeviceBasis(n, h, -1, S)
+    db.build()
+    reps = db.states
+    diag, off = expr.split()
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(reps.size)
+    want = O.matvec_push(og, n, -1, reps, O.Terms(diag), O.Terms(off), x)
+    X = rng.standard_normal((reps.size, 8))
+    ref8 = np.stack([O.matvec_push(og, n, -1, reps, O.Terms(diag), O.Terms(off), X[:, q]) for q in range(8)], 1)
+    dop = lsamd.DeviceOperator(db, expr)
+    dop.build_csr(96)
+    for flags in range(4):
+        for lanes in (1, 8, 64):
+            say("schedule", lanes, flags)
+            dop.set_schedule(lanes, flags, lanes, flags)
+            y = dop.matvec(x, "csr")
+            assert np.abs(y - want).max() <= RTOL64 * np.abs(want).max()
+            Y32 = dop.matvec_block(X, 32)
+            assert np.abs(Y32 - ref8).max() <= 2e-5 * np.abs(ref8).max()
+            Y64 = dop.matvec_block(X, 64)
+            assert np.abs(Y64 - ref8).max() <= RTOL64 * np.abs(ref8).max()
+    dop.autotune()
+    sched = dop.schedule
+    assert sched["persistent_blocks"] % 8 == 0
+    assert np.abs(dop.matvec(x, "csr") - want).max() <= RTOL64 * np.abs(want).max()
+    with pytest.raises(lsamd.EngineError):
+        dop.set_schedule(3, 0, 8, 0)
+
+
 def test_matvec_rejects_operator_breaking_symmetry(engine):
     say("error path")
     n, h = 12, 6
