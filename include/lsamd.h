@@ -378,6 +378,12 @@ int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size,
                             uint64_t seed, double *evals, double *residuals,
                             double *d_ritz_out, ls_amd_eig_info *info);
 
+/* Host utility: eigen-decomposition of a dense symmetric n x n matrix A
+   (row-major) by Householder tridiagonalisation and implicit QR with
+   Wilkinson shifts, as the Krylov solvers use for their projected
+   matrices.  w[n] ascending; Z (may be NULL) the eigenvectors as columns. */
+int ls_amd_dense_eigh(int n, double const *A, double *w, double *Z);
+
 /* ---- Many symmetry sectors in one product --------------------------------
    No counterpart in the reference, which builds one basis per sector and
    diagonalises the sectors one after another (chapel/src/Diagonalize.chpl:

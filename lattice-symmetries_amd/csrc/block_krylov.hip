@@ -16,7 +16,7 @@
 // cycle each segment's projected matrix is diagonalised on a host thread and
 // the lowest `keep` Ritz blocks plus the residual block restart the next
 // cycle.  The product runs in FP32 until the relative residual falls below
-// switch_tol (or stops improving), then the process restarts explicitly from
+// switch_tol (or stops improving by 10 % per cycle), then the process restarts explicitly from
 // the lowest Ritz block with the FP64 product.  Final eigenvalues and
 // residuals are FP64 Rayleigh quotients and residual norms of the Ritz
 // vectors.
@@ -332,54 +332,6 @@ struct BlockAlgebra {
 };
 
 
-void jacobi_eigh_local(std::vector<double> A, int m, std::vector<double> &w, std::vector<double> &Z) {
-  Z.assign(static_cast<size_t>(m) * m, 0.0);
-  for (int i = 0; i < m; ++i) Z[static_cast<size_t>(i) * m + i] = 1.0;
-  auto at = [&](int i, int j) -> double & { return A[static_cast<size_t>(i) * m + j]; };
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < m; ++i)
-      for (int j = 0; j < m; ++j) {
-        tot += at(i, j) * at(i, j);
-        if (i != j) off += at(i, j) * at(i, j);
-      }
-    if (off <= 1e-30 * tot || off == 0.0) break;
-    for (int p = 0; p < m - 1; ++p)
-      for (int q = p + 1; q < m; ++q) {
-        double const apq = at(p, q);
-        if (std::fabs(apq) < 1e-300) continue;
-        double const theta = (at(q, q) - at(p, p)) / (2.0 * apq);
-        double const t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        double const c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < m; ++k) {
-          double const akp = at(k, p), akq = at(k, q);
-          at(k, p) = c * akp - s * akq;
-          at(k, q) = s * akp + c * akq;
-        }
-        for (int k = 0; k < m; ++k) {
-          double const apk = at(p, k), aqk = at(q, k);
-          at(p, k) = c * apk - s * aqk;
-          at(q, k) = s * apk + c * aqk;
-        }
-        for (int k = 0; k < m; ++k) {
-          double const zkp = Z[static_cast<size_t>(k) * m + p], zkq = Z[static_cast<size_t>(k) * m + q];
-          Z[static_cast<size_t>(k) * m + p] = c * zkp - s * zkq;
-          Z[static_cast<size_t>(k) * m + q] = s * zkp + c * zkq;
-        }
-      }
-  }
-  std::vector<int> order(m);
-  for (int i = 0; i < m; ++i) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](int a, int b) { return at(a, a) < at(b, b); });
-  w.resize(m);
-  std::vector<double> Zs(static_cast<size_t>(m) * m);
-  for (int c = 0; c < m; ++c) {
-    w[c] = at(order[c], order[c]);
-    for (int r = 0; r < m; ++r) Zs[static_cast<size_t>(r) * m + c] = Z[static_cast<size_t>(r) * m + order[c]];
-  }
-  Z.swap(Zs);
-}
-
 __global__ __launch_bounds__(kThreads) void column_kernel(double const *__restrict__ X, int b, int col,
                                                           double *__restrict__ out, int64_t rows) {
   int64_t const i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
@@ -523,8 +475,10 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, 
       }
     }
     // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
+    std::vector<int> eig_ok(ns, 1);
     parallel_for(ns, [&](int s) {
-      jacobi_eigh_local(T[s], dimT, theta[s], Z[s]);
+      eig_ok[s] = symmetric_eigh(T[s], dimT, theta[s], Z[s]) ? 1 : 0;
+      if (!eig_ok[s]) return;
       double w = 0.0;
       double const *Rs = Rf.data() + static_cast<size_t>(s) * V;
       for (int q = 0; q < want; ++q) {
@@ -538,6 +492,8 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, 
       }
       worst[s] = w;
     });
+    for (int s = 0; s < ns; ++s)
+      if (!eig_ok[s]) return fail("projected eigenproblem did not converge in segment " + std::to_string(s));
     double wmax = 0.0;
     for (int s = 0; s < ns; ++s) wmax = std::max(wmax, worst[s]);
     ++cycles_run;
@@ -548,7 +504,7 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, 
     }
     if (cycle + 1 == o.max_cycles) break; // keep V and Z for the final Ritz block
     // FP32 gives way to FP64 at the switch tolerance or when FP32 stalls
-    bool const to_fp64 = fp32 && (wmax < o.switch_tol || wmax > 0.5 * prev_worst);
+    bool const to_fp64 = fp32 && (wmax < o.switch_tol || wmax > 0.9 * prev_worst);
     prev_worst = wmax;
     // Restart: Y_t = sum_i V_i Z_s[i-block, t-block]
     int const r = to_fp64 ? 1 : o.keep;

@@ -119,6 +119,10 @@ using BlockProduct = std::function<bool(bool fp32, double const *x, double *y)>;
 bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, BlockProduct const &product,
                         BlockKrylovOptions const &o, BlockKrylovResult &res, double *d_ritz_block);
 bool block_to_f32(double const *x, float *y, int64_t count);
+
+// Dense symmetric eigen-decomposition (dense_eigh.cpp): A is n x n row-major;
+// w receives the eigenvalues ascending and Z the eigenvectors as columns.
+bool symmetric_eigh(std::vector<double> A, int n, std::vector<double> &w, std::vector<double> &Z);
 bool block_column(double const *X, int b, int col, double *out, int64_t rows);
 
 } // namespace lsamd

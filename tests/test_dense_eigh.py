@@ -1,0 +1,73 @@
+"""Host dense symmetric eigensolver used by the Krylov solvers
+(ls_amd_dense_eigh: Householder tridiagonalisation + implicit QR), against
+numpy.linalg.eigh and the defining identities."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from lsamd import _abi
+
+
+def eigh(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.zeros(max(n, 1))
+    Z = np.zeros((max(n, 1), max(n, 1)))
+    p = C.POINTER(C.c_double)
+    rc = _abi.load().ls_amd_dense_eigh(n, A.ctypes.data_as(p), w.ctypes.data_as(p), Z.ctypes.data_as(p))
+    assert rc == 0, _abi.last_error()
+    return w[:n], Z[:n, :n]
+
+
+def check(A, tol=1e-12):
+    w, Z = eigh(A)
+    n = A.shape[0]
+    scale = max(1.0, np.abs(A).max() * n)
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - np.linalg.eigvalsh(A)).max() <= tol * scale
+    assert np.abs(A @ Z - Z * w).max() <= 1e-11 * scale
+    assert np.abs(Z.T @ Z - np.eye(n)).max() <= 1e-12 * n
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 64, 160])
+def test_random_symmetric(n):
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((n, n))
+    check(M + M.T)
+
+
+def test_structured_matrices():
+    rng = np.random.default_rng(5)
+    # real embedding of a complex Hermitian matrix: every eigenvalue twice
+    A = rng.standard_normal((30, 30))
+    B = rng.standard_normal((30, 30))
+    Hr, Hi = A + A.T, B - B.T
+    R = np.block([[Hr, -Hi], [Hi, Hr]])
+    w, _ = eigh(R)
+    assert np.allclose(w[0::2], w[1::2], atol=1e-10)
+    check(R)
+    # already diagonal / tridiagonal / block diagonal with exact zeros
+    check(np.diag(rng.standard_normal(40)))
+    T = np.diag(rng.standard_normal(40)) + np.diag(rng.standard_normal(39), 1)
+    check(T + np.triu(T, 1).T)
+    C2 = np.zeros((50, 50))
+    C2[:20, :20] = (lambda m: m + m.T)(rng.standard_normal((20, 20)))
+    C2[20:, 20:] = (lambda m: m + m.T)(rng.standard_normal((30, 30)))
+    check(C2)
+    # highly degenerate: rank-one update of the identity, and the zero matrix
+    v = rng.standard_normal(25)
+    check(np.eye(25) * 3.0 + np.outer(v, v))
+    check(np.zeros((12, 12)))
+    # block-tridiagonal projected matrix of a block Lanczos run (widely spread spectrum)
+    blocks = 12
+    b = 8
+    T = np.zeros((blocks * b, blocks * b))
+    for j in range(blocks):
+        Aj = rng.standard_normal((b, b))
+        T[j * b:(j + 1) * b, j * b:(j + 1) * b] = (Aj + Aj.T) * 10 - 40 * np.eye(b)
+        if j + 1 < blocks:
+            Bj = np.triu(rng.standard_normal((b, b)))
+            T[(j + 1) * b:(j + 2) * b, j * b:(j + 1) * b] = Bj
+            T[j * b:(j + 1) * b, (j + 1) * b:(j + 2) * b] = Bj.T
+    check(T)
