@@ -247,7 +247,11 @@ def main():
     sched = op.schedule
     log(f"autotune: {sched}")
     nt, die = sched["flags_f64"] & 1, (sched["flags_f64"] >> 1) & 1
-    kernel_name = f"csr_f64_kernel<{lanes}, {'true' if nt else 'false'}, {'true' if die else 'false'}>"
+    csr_info = op.csr_info
+    packed = csr_info["packed"]
+    tf = lambda b: "true" if b else "false"  # noqa: E731
+    kernel_name = f"csr_f64_kernel<{lanes}, {tf(nt)}, {tf(die)}, {tf(packed)}>"
+    log(f"csr: value table {'of ' + str(csr_info['table_size']) + ' values' if packed else 'not used'}")
 
     rng = np.random.default_rng(42)
     x = lsamd.DeviceArray.from_numpy(rng.standard_normal(dim))
@@ -329,7 +333,7 @@ def main():
         "config": {"workload": f"J1-J2 {args.lattice}x{args.lattice} square lattice, J2=0.5, Hamming weight {h}, "
                                "T x C4v x Z2 (576 elements), FP64 CSR H x",
                    "states": dim, "nonzeros": nnz, "off_diagonal_nonzeros": nnz_off,
-                   "schedule": sched},
+                   "schedule": sched, "value_table": csr_info["table_size"] if packed else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": kernel_name, "kernel_ms": k_ms,

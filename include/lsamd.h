@@ -287,6 +287,14 @@ int ls_amd_matvec_c128(ls_amd_operator *op, double const *d_x, double *d_y);
    and FP32, for sector batches (the real-valued CSR products reject it).
    Returns the number of stored off-diagonal nonzeros or -1. */
 int64_t ls_amd_operator_build_csr(ls_amd_operator *op, int precision);
+/* Value table (default on): a real CSR with at most 256 distinct stored
+   values and fewer than 2^24 rows keeps each entry as one 32-bit word,
+   column | value index << 24, and the values in a table (set before
+   ls_amd_operator_build_csr). */
+int ls_amd_operator_set_csr_packing(ls_amd_operator *op, int enable);
+/* out[4]: stored off-diagonal nonzeros, packed (0/1), value-table size,
+   complex (0/1). */
+int ls_amd_operator_csr_info(ls_amd_operator const *op, int64_t *out);
 int64_t ls_amd_operator_csr_nnz(ls_amd_operator const *op);
 /* Stored CSR (off-diagonal) rows: row_ptr[n+1] (int64), cols, values
    (2 nnz interleaved re, im for a complex CSR). */

@@ -741,6 +741,19 @@ extern "C" int ls_amd_operator_copy_csr(ls_amd_operator const *op, int64_t *row_
   Csr const &c = op->core.csr;
   if (!c.has64) return fail("FP64 CSR has not been built") ? 0 : -1;
   size_t const width = c.is_complex ? 2 : 1;
+  if (c.packed) {
+    std::vector<double> table(static_cast<size_t>(c.table_size));
+    bool const ok = to_host(row_ptr, c.row_ptr.ptr, static_cast<size_t>(c.rows + 1)) &&
+                    to_host(cols, c.cols.ptr, static_cast<size_t>(c.nnz)) &&
+                    to_host(table.data(), c.table64.ptr, table.size()) &&
+                    to_host(diag, c.diag64.ptr, static_cast<size_t>(c.rows));
+    if (!ok) return -1;
+    for (int64_t k = 0; k < c.nnz; ++k) {
+      values[k] = table[cols[k] >> 24];
+      cols[k] &= kPackedColumnMask;
+    }
+    return 0;
+  }
   bool const ok = to_host(row_ptr, c.row_ptr.ptr, static_cast<size_t>(c.rows + 1)) &&
                   to_host(cols, c.cols.ptr, static_cast<size_t>(c.nnz)) &&
                   to_host(values, c.vals64.ptr, width * static_cast<size_t>(c.nnz)) &&
@@ -764,6 +777,22 @@ extern "C" int ls_amd_csr_matvec_f64_block(ls_amd_operator *op, int k, double co
 }
 
 extern "C" int ls_amd_operator_autotune(ls_amd_operator *op) { return operator_autotune(op->core); }
+
+extern "C" int ls_amd_operator_set_csr_packing(ls_amd_operator *op, int enable) {
+  if (op == nullptr) return fail("null argument") ? 0 : -1;
+  op->core.allow_packing = enable != 0;
+  return 0;
+}
+
+extern "C" int ls_amd_operator_csr_info(ls_amd_operator const *op, int64_t *out) {
+  if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
+  Csr const &c = op->core.csr;
+  out[0] = c.nnz;
+  out[1] = c.packed ? 1 : 0;
+  out[2] = c.table_size;
+  out[3] = c.is_complex ? 1 : 0;
+  return 0;
+}
 
 extern "C" int ls_amd_operator_schedule(ls_amd_operator const *op, int *out) {
   if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;

@@ -479,6 +479,128 @@ __global__ __launch_bounds__(256) void to_f32_kernel(int64_t n, double const *__
   if (i < n) out[i] = static_cast<float>(in[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Value table: distinct off-diagonal values (bit patterns) through a per-block
+// LDS hash merged into a global hash, then column | index << 24 packing.
+// ---------------------------------------------------------------------------
+
+constexpr uint64_t kEmptySlot = ~uint64_t{0}; // a NaN pattern, never a stored value
+constexpr int kLocalSlots = 512;
+constexpr int kGlobalSlots = 4096;
+
+__device__ __forceinline__ uint32_t slot_hash(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  return static_cast<uint32_t>(k);
+}
+
+__global__ __launch_bounds__(256) void distinct_values_kernel(double const *__restrict__ vals, int64_t nnz,
+                                                              unsigned long long *__restrict__ table,
+                                                              int *__restrict__ overflow) {
+  __shared__ unsigned long long local[kLocalSlots];
+  __shared__ int local_overflow;
+  for (int i = threadIdx.x; i < kLocalSlots; i += blockDim.x) local[i] = kEmptySlot;
+  if (threadIdx.x == 0) local_overflow = 0;
+  __syncthreads();
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz;
+       k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    unsigned long long const key = static_cast<unsigned long long>(__double_as_longlong(vals[k]));
+    uint32_t slot = slot_hash(key) & (kLocalSlots - 1);
+    int probes = 0;
+    while (true) {
+      unsigned long long const prev = atomicCAS(&local[slot], static_cast<unsigned long long>(kEmptySlot), key);
+      if (prev == kEmptySlot || prev == key) break;
+      slot = (slot + 1) & (kLocalSlots - 1);
+      if (++probes >= kLocalSlots) {
+        local_overflow = 1;
+        break;
+      }
+    }
+    if (local_overflow) break;
+  }
+  __syncthreads();
+  if (local_overflow) {
+    if (threadIdx.x == 0) atomicExch(overflow, 1);
+    return;
+  }
+  for (int i = threadIdx.x; i < kLocalSlots; i += blockDim.x) {
+    unsigned long long const key = local[i];
+    if (key == kEmptySlot) continue;
+    uint32_t slot = slot_hash(key) & (kGlobalSlots - 1);
+    int probes = 0;
+    while (true) {
+      unsigned long long const prev = atomicCAS(&table[slot], static_cast<unsigned long long>(kEmptySlot), key);
+      if (prev == kEmptySlot || prev == key) break;
+      slot = (slot + 1) & (kGlobalSlots - 1);
+      if (++probes >= kGlobalSlots) {
+        atomicExch(overflow, 1);
+        break;
+      }
+    }
+  }
+}
+
+// cols[k] |= index of vals[k] in the sorted table << 24
+__global__ __launch_bounds__(256) void pack_columns_kernel(double const *__restrict__ vals, int64_t nnz,
+                                                           double const *__restrict__ table, int size,
+                                                           uint32_t *__restrict__ cols) {
+  __shared__ double t[kValueTableSize];
+  for (int i = threadIdx.x; i < size; i += blockDim.x) t[i] = table[i];
+  __syncthreads();
+  int64_t const k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  double const v = vals[k];
+  int lo = 0, hi = size - 1;
+  while (lo < hi) {
+    int const mid = (lo + hi) >> 1;
+    if (t[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  cols[k] = (cols[k] & kPackedColumnMask) | (static_cast<uint32_t>(lo) << 24);
+}
+
+// Replaces the value arrays of a real CSR by a value table when it has at
+// most 256 distinct values and fewer than 2^24 rows; vals is the FP64 value
+// array (kept by the caller until this returns).
+bool try_pack_values(Csr &c, double const *vals) {
+  if (c.rows >= (int64_t{1} << 24) || c.nnz == 0) return true;
+  DeviceArray<unsigned long long> table;
+  DeviceArray<int> overflow;
+  if (!table.allocate(kGlobalSlots) || !overflow.allocate(1)) return false;
+  LSAMD_HIP(hipMemsetAsync(table.ptr, 0xFF, sizeof(unsigned long long) * kGlobalSlots, stream()));
+  LSAMD_HIP(hipMemsetAsync(overflow.ptr, 0, sizeof(int), stream()));
+  distinct_values_kernel<<<2048, 256, 0, stream()>>>(vals, c.nnz, table.ptr, overflow.ptr);
+  LSAMD_HIP(hipGetLastError());
+  std::vector<unsigned long long> h(kGlobalSlots);
+  int ovf = 0;
+  LSAMD_HIP(hipMemcpyAsync(h.data(), table.ptr, sizeof(unsigned long long) * kGlobalSlots, hipMemcpyDeviceToHost,
+                           stream()));
+  LSAMD_HIP(hipMemcpyAsync(&ovf, overflow.ptr, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  if (ovf) return true;
+  std::vector<double> values;
+  for (unsigned long long k : h)
+    if (k != kEmptySlot) {
+      double v;
+      std::memcpy(&v, &k, sizeof(v));
+      values.push_back(v);
+    }
+  if (values.empty() || values.size() > static_cast<size_t>(kValueTableSize)) return true;
+  std::sort(values.begin(), values.end());
+  std::vector<float> values32(values.size());
+  for (size_t i = 0; i < values.size(); ++i) values32[i] = static_cast<float>(values[i]);
+  if (!c.table64.upload(values.data(), values.size()) || !c.table32.upload(values32.data(), values32.size()))
+    return false;
+  pack_columns_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(vals, c.nnz, c.table64.ptr,
+                                                                  static_cast<int>(values.size()), c.cols.ptr);
+  LSAMD_HIP(hipGetLastError());
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  c.packed = true;
+  c.table_size = static_cast<int>(values.size());
+  return true;
+}
+
 bool operator_build_csr(OperatorCore &op, int precision) {
   if (precision != 32 && precision != 64 && precision != 96 && precision != 128)
     return fail("precision must be 32, 64, 96 (real) or 128 (complex)");
@@ -576,16 +698,19 @@ bool operator_build_csr(OperatorCore &op, int precision) {
     LSAMD_HIP(hipStreamSynchronize(stream()));
     return true;
   }
+  if (op.allow_packing && !try_pack_values(c, vals.ptr)) return false;
   if (precision == 32 || precision == 96) {
-    if (!c.vals32.allocate(static_cast<size_t>(c.nnz > 0 ? c.nnz : 1))) return false;
     if (!c.diag32.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
-    if (c.nnz > 0) to_f32_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(c.nnz, vals.ptr, c.vals32.ptr);
+    if (!c.packed) {
+      if (!c.vals32.allocate(static_cast<size_t>(c.nnz > 0 ? c.nnz : 1))) return false;
+      if (c.nnz > 0) to_f32_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(c.nnz, vals.ptr, c.vals32.ptr);
+    }
     if (n > 0) to_f32_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, c.diag64.ptr, c.diag32.ptr);
     LSAMD_HIP(hipGetLastError());
     c.has32 = true;
   }
   if (precision == 64 || precision == 96) {
-    c.vals64 = std::move(vals);
+    if (!c.packed) c.vals64 = std::move(vals);
     c.has64 = true;
   } else {
     c.diag64.reset();
@@ -627,14 +752,44 @@ template <bool DIE> __device__ __forceinline__ void tile_range(int64_t tiles, in
   }
 }
 
-template <int LPR, bool NT, bool DIE>
+// One stored entry: (column, value), from separate arrays or from a packed
+// column word and the value table in LDS.
+template <bool PACKED, bool NT, typename T>
+__device__ __forceinline__ void csr_entry(uint32_t const *__restrict__ cols, T const *__restrict__ vals,
+                                          T const *__restrict__ table, int64_t k, uint32_t &col, T &v) {
+  if constexpr (PACKED) {
+    uint32_t const p = ld_stream<NT>(cols + k);
+    col = p & kPackedColumnMask;
+    v = table[p >> 24];
+  } else {
+    col = ld_stream<NT>(cols + k);
+    v = ld_stream<NT>(vals + k);
+  }
+}
+
+// Stages the value table in LDS (a no-op for unpacked matrices).
+template <bool PACKED, typename T>
+__device__ __forceinline__ T const *stage_table(T *lds, T const *__restrict__ table, int size) {
+  if constexpr (PACKED) {
+    for (int i = threadIdx.x; i < size; i += blockDim.x) lds[i] = table[i];
+    __syncthreads();
+    return lds;
+  } else {
+    return nullptr;
+  }
+}
+
+template <int LPR, bool NT, bool DIE, bool PACKED>
 __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                       uint32_t const *__restrict__ cols,
                                                       double const *__restrict__ vals,
+                                                      double const *__restrict__ table, int table_size,
                                                       double const *__restrict__ diag,
                                                       double const *__restrict__ x,
                                                       double *__restrict__ y, int accumulate,
                                                       int has_diag) {
+  __shared__ double lds_table[PACKED ? kValueTableSize : 1];
+  double const *tab = stage_table<PACKED>(lds_table, table, table_size);
   constexpr int kRows = 256 / LPR;
   int64_t const tiles = (n + kRows - 1) / kRows;
   int64_t t, t_end, t_step;
@@ -645,7 +800,12 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
     double acc = 0.0;
     if (row < n) {
       int64_t const b = rp[row], e = rp[row + 1];
-      for (int64_t k = b + sub; k < e; k += LPR) acc = fma(ld_stream<NT>(vals + k), x[ld_stream<NT>(cols + k)], acc);
+      for (int64_t k = b + sub; k < e; k += LPR) {
+        uint32_t col;
+        double v;
+        csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
+        acc = fma(v, x[col], acc);
+      }
     }
 #pragma unroll
     for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
@@ -656,13 +816,16 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
   }
 }
 
-template <int K, int LPR, typename T, typename TO, bool NT, bool DIE>
+template <int K, int LPR, typename T, typename TO, bool NT, bool DIE, bool PACKED>
 __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                         uint32_t const *__restrict__ cols,
                                                         T const *__restrict__ vals,
+                                                        T const *__restrict__ table, int table_size,
                                                         T const *__restrict__ diag,
                                                         T const *__restrict__ x,
                                                         TO *__restrict__ y) {
+  __shared__ T lds_table[PACKED ? kValueTableSize : 1];
+  T const *tab = stage_table<PACKED>(lds_table, table, table_size);
   constexpr int kRows = 256 / LPR;
   int64_t const tiles = (n + kRows - 1) / kRows;
   int64_t t, t_end, t_step;
@@ -676,8 +839,10 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const
     if (row < n) {
       int64_t const b = rp[row], e = rp[row + 1];
       for (int64_t k = b + sub; k < e; k += LPR) {
-        T const v = ld_stream<NT>(vals + k);
-        T const *xr = x + static_cast<size_t>(ld_stream<NT>(cols + k)) * K;
+        uint32_t col;
+        T v;
+        csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
+        T const *xr = x + static_cast<size_t>(col) * K;
 #pragma unroll
         for (int q = 0; q < K; ++q) acc[q] = fma(v, xr[q], acc[q]);
       }
@@ -745,9 +910,14 @@ bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool acc
   int const grid = product_grid(op, n, op.lanes_f64, op.flags_f64);
   bool ok = dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
     return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
-      csr_f64_kernel<LPR, NT, DIE><<<grid, 256, 0, stream()>>>(n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr,
-                                                               c.diag64.ptr, d_x, d_y, accumulate ? 1 : 0,
-                                                               op.diag.count > 0 ? 1 : 0);
+      if (c.packed)
+        csr_f64_kernel<LPR, NT, DIE, true><<<grid, 256, 0, stream()>>>(
+            n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y,
+            accumulate ? 1 : 0, op.diag.count > 0 ? 1 : 0);
+      else
+        csr_f64_kernel<LPR, NT, DIE, false><<<grid, 256, 0, stream()>>>(
+            n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, nullptr, 0, c.diag64.ptr, d_x, d_y, accumulate ? 1 : 0,
+            op.diag.count > 0 ? 1 : 0);
       LSAMD_HIP(hipGetLastError());
       return true;
     });
@@ -767,8 +937,12 @@ bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_
     return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
       int const grid = product_grid(op, n, LPR, op.flags_f32);
       return dispatch_flags(op.flags_f32, [&]<bool NT, bool DIE>() -> bool {
-        csr_block_kernel<K, LPR, float, float, NT, DIE><<<grid, 256, 0, stream()>>>(
-            n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
+        if (c.packed)
+          csr_block_kernel<K, LPR, float, float, NT, DIE, true><<<grid, 256, 0, stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table32.ptr, c.table_size, c.diag32.ptr, d_x, d_y);
+        else
+          csr_block_kernel<K, LPR, float, float, NT, DIE, false><<<grid, 256, 0, stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, nullptr, 0, c.diag32.ptr, d_x, d_y);
         LSAMD_HIP(hipGetLastError());
         return true;
       });
@@ -789,8 +963,12 @@ bool operator_csr_f32_block_out64(OperatorCore &op, int k, float const *d_x, dou
     return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
       int const grid = product_grid(op, n, LPR, op.flags_f32);
       return dispatch_flags(op.flags_f32, [&]<bool NT, bool DIE>() -> bool {
-        csr_block_kernel<K, LPR, float, double, NT, DIE><<<grid, 256, 0, stream()>>>(
-            n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, d_x, d_y);
+        if (c.packed)
+          csr_block_kernel<K, LPR, float, double, NT, DIE, true><<<grid, 256, 0, stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table32.ptr, c.table_size, c.diag32.ptr, d_x, d_y);
+        else
+          csr_block_kernel<K, LPR, float, double, NT, DIE, false><<<grid, 256, 0, stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, nullptr, 0, c.diag32.ptr, d_x, d_y);
         LSAMD_HIP(hipGetLastError());
         return true;
       });
@@ -811,8 +989,12 @@ bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *
     return dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
       int const grid = product_grid(op, n, LPR, op.flags_f64);
       return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
-        csr_block_kernel<K, LPR, double, double, NT, DIE><<<grid, 256, 0, stream()>>>(
-            n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, d_x, d_y);
+        if (c.packed)
+          csr_block_kernel<K, LPR, double, double, NT, DIE, true><<<grid, 256, 0, stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y);
+        else
+          csr_block_kernel<K, LPR, double, double, NT, DIE, false><<<grid, 256, 0, stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, nullptr, 0, c.diag64.ptr, d_x, d_y);
         LSAMD_HIP(hipGetLastError());
         return true;
       });

@@ -35,7 +35,17 @@ struct Csr {
   DeviceArray<float> diag32;
   bool has64 = false, has32 = false;
   bool is_complex = false; // values are (re, im) pairs of M[i, j]
+  // Value table: when the matrix has at most 256 distinct off-diagonal
+  // values and fewer than 2^24 rows, cols[k] holds column | (value index
+  // << 24) and vals64 / vals32 are replaced by table64 / table32.
+  bool packed = false;
+  int table_size = 0;
+  DeviceArray<double> table64;
+  DeviceArray<float> table32;
 };
+
+constexpr uint32_t kPackedColumnMask = (1u << 24) - 1;
+constexpr int kValueTableSize = 256;
 
 struct OperatorCore {
   // basis side
@@ -57,6 +67,7 @@ struct OperatorCore {
   int flags_f64 = 0;  // bit 0 non-temporal stream loads, bit 1 die-aware persistent grid
   int flags_f32 = 0;
   int persistent_blocks = 1024; // grid of the die-aware schedule (a multiple of 8)
+  bool allow_packing = true;    // value-table CSR when possible
   DeviceArray<int> d_err; // error flag + offending state
   DeviceArray<uint64_t> d_err_state;
 };

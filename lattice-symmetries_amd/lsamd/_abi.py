@@ -231,6 +231,8 @@ SIGNATURES = {
     "ls_amd_csr_matvec_f32": (C.c_int, [_P, C.c_int, _P, _P]),
     "ls_amd_csr_matvec_f64_block": (C.c_int, [_P, C.c_int, _P, _P]),
     "ls_amd_operator_autotune": (C.c_int, [_P]),
+    "ls_amd_operator_set_csr_packing": (C.c_int, [_P, C.c_int]),
+    "ls_amd_operator_csr_info": (C.c_int, [_P, _i64p]),
     "ls_amd_operator_schedule": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "ls_amd_operator_set_schedule": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "ls_amd_last_kernel_ms": (C.c_double, [C.c_int]),

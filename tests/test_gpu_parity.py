@@ -234,8 +234,20 @@ def test_every_csr_schedule_gives_the_same_product(engine):
     want = O.matvec_push(og, n, -1, reps, O.Terms(diag), O.Terms(off), x)
     X = rng.standard_normal((reps.size, 8))
     ref8 = np.stack([O.matvec_push(og, n, -1, reps, O.Terms(diag), O.Terms(off), X[:, q]) for q in range(8)], 1)
+    # the value-table CSR (default) and the plain CSR store the same matrix
+    plain = lsamd.DeviceOperator(db, expr)
+    plain.set_csr_packing(False)
+    plain.build_csr(96)
+    assert not plain.csr_info["packed"]
     dop = lsamd.DeviceOperator(db, expr)
     dop.build_csr(96)
+    info = dop.csr_info
+    say("csr info", info)
+    assert info["packed"] and 0 < info["table_size"] <= 256
+    for a, b in zip(dop.csr_arrays(), plain.csr_arrays()):
+        assert np.array_equal(a, b)
+    assert np.abs(plain.matvec(x, "csr") - want).max() <= RTOL64 * np.abs(want).max()
+    assert np.abs(plain.matvec_block(X, 32) - ref8).max() <= 2e-5 * np.abs(ref8).max()
     for flags in range(4):
         for lanes in (1, 8, 64):
             say("schedule", lanes, flags)
