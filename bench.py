@@ -100,8 +100,8 @@ def sectors_bench(args):
     t_setup = time.perf_counter() - t0
     dim, nnz_off = batch.number_states, batch.nnz
     log(f"sectors: {batch.count} sectors, {dim} states, {nnz_off} off-diagonal nonzeros, set-up {t_setup:.1f} s")
-    keep = set(args.reference_sectors.split(","))
     names = [f"{k}{'+' if si == 1 else '-'}" for k, si in batch.sectors]
+    keep = set(names) if args.reference_sectors == "all" else set(args.reference_sectors.split(","))
     ref_ops = {nm: op for nm, op in zip(names, ops) if nm in keep}
     del bases, ops
     sched = batch.autotune(8)
@@ -191,6 +191,34 @@ def copy_calibration(args):
                       "copies": args.steps}), flush=True)
 
 
+def gather_ceiling(args):
+    """The FP64 product kernel on synthetic matrices of the 6x6 workload's
+    shape (15 804 956 rows x 74 entries) whose rows gather from windows of W
+    columns: small windows keep the gathered vector in L2, W = rows spreads
+    the gathers over the whole vector.  The best time at small W bounds what
+    any reordering of the real matrix could reach with one gather per
+    nonzero."""
+    import ctypes as C
+    import lsamd
+    lib = lsamd.load()
+    rows, per_row = 15804956, 74
+    out = []
+    for packed in (1, 0):
+        for window in (1 << 16, 1 << 18, 1 << 20, 1 << 22, rows):
+            for lanes in (8, 16, 32):
+                ms = C.c_double()
+                rc = lib.ls_amd_gather_probe(rows, per_row, window, lanes, packed, 5, C.byref(ms))
+                if rc != 0:
+                    raise SystemExit(lsamd._abi.last_error())
+                nnz = rows * per_row
+                out.append({"packed": packed, "window_columns": window, "window_MB": window * 8 / 1e6,
+                             "lanes": lanes, "ms": ms.value, "gathers_per_s": nnz / (ms.value * 1e-3),
+                             "algorithmic_GBs": (20.0 * nnz + 32.0 * rows) / (ms.value * 1e-3) / 1e9})
+                log(f"packed {packed} window {window:9d} lanes {lanes:2d}: {ms.value:.3f} ms, "
+                    f"{nnz / (ms.value * 1e-3) / 1e9:.1f} G gathers/s")
+    print(json.dumps({"workload": "gather-ceiling", "rows": rows, "per_row": per_row, "results": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,7 +230,8 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--block-cycle", type=int, default=12)
-    ap.add_argument("--workload", choices=["square6", "chain-sectors", "copy-calibration"], default="square6")
+    ap.add_argument("--workload", choices=["square6", "chain-sectors", "copy-calibration", "gather-ceiling"],
+                    default="square6")
     ap.add_argument("--copy-mib", type=int, default=2048)
     ap.add_argument("--chain", type=int, default=28)
     ap.add_argument("--sector-blocks", type=int, default=12)
@@ -225,6 +254,9 @@ def main():
         raise SystemExit("no usable gfx950 device: " + _abi.last_error())
     if args.workload == "chain-sectors":
         sectors_bench(args)
+        return
+    if args.workload == "gather-ceiling":
+        gather_ceiling(args)
         return
 
     n, h, S, expr = workload(args.lattice)

@@ -1078,3 +1078,88 @@ int operator_autotune(OperatorCore &op) {
 }
 
 } // namespace lsamd
+
+// ---------------------------------------------------------------------------
+// Gather-rate probe: the FP64 product kernel on a synthetic matrix whose
+// rows gather from a window of `window` columns (rows of one 65 536-row block
+// share a window; window = rows draws from the whole vector).  Measures how
+// fast the product can go when its gathers are served from L2 (small
+// windows) versus from memory, with everything else as in the real product.
+// ---------------------------------------------------------------------------
+
+namespace lsamd {
+namespace {
+
+__global__ __launch_bounds__(256) void probe_rows_kernel(int64_t rows, int per_row, int64_t *__restrict__ rp) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i <= rows) rp[i] = i * per_row;
+}
+
+__global__ __launch_bounds__(256) void probe_cols_kernel(int64_t rows, int per_row, int64_t window, int packed,
+                                                         uint32_t *__restrict__ cols) {
+  int64_t const k = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (k >= rows * per_row) return;
+  uint64_t z = 0x9e3779b97f4a7c15ull * static_cast<uint64_t>(k + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  int64_t const row = k / per_row;
+  uint64_t const span = static_cast<uint64_t>(rows - window);
+  uint64_t const base = window >= rows ? 0 : (static_cast<uint64_t>(row / 65536) * 2654435761ull) % (span + 1);
+  uint32_t const col = static_cast<uint32_t>(base + z % static_cast<uint64_t>(window));
+  cols[k] = packed ? (col | (static_cast<uint32_t>(z >> 40) & 0x3Fu) << 24) : col;
+}
+
+} // namespace
+} // namespace lsamd
+
+extern "C" int ls_amd_gather_probe(int64_t rows, int per_row, int64_t window, int lanes, int packed, int reps,
+                                   double *ms_out) {
+  using namespace lsamd;
+  if (ms_out == nullptr || rows <= 0 || per_row <= 0 || window <= 0 || window > rows || reps <= 0)
+    return fail("ls_amd_gather_probe: bad arguments") ? 0 : -1;
+  if (rows >= (int64_t{1} << 24) && packed) return fail("ls_amd_gather_probe: packed needs rows < 2^24") ? 0 : -1;
+  if (rows >= (int64_t{1} << 32) - 1) return fail("ls_amd_gather_probe: too many rows") ? 0 : -1;
+  if (!ensure_device()) return -1;
+  OperatorCore op;
+  Csr &c = op.csr;
+  c.rows = rows;
+  c.nnz = rows * per_row;
+  DeviceArray<double> x, y;
+  if (!c.row_ptr.allocate(static_cast<size_t>(rows + 1)) || !c.cols.allocate(static_cast<size_t>(c.nnz)) ||
+      !c.diag64.allocate(static_cast<size_t>(rows)) || !x.allocate(static_cast<size_t>(rows)) ||
+      !y.allocate(static_cast<size_t>(rows)))
+    return -1;
+  probe_rows_kernel<<<grid_for(rows + 1, 256), 256, 0, stream()>>>(rows, per_row, c.row_ptr.ptr);
+  probe_cols_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(rows, per_row, window, packed, c.cols.ptr);
+  if (hipGetLastError() != hipSuccess) return fail("ls_amd_gather_probe: launch failed") ? 0 : -1;
+  if (hipMemsetAsync(x.ptr, 0, sizeof(double) * rows, stream()) != hipSuccess) return -1;
+  if (packed) {
+    std::vector<double> table(64);
+    for (int i = 0; i < 64; ++i) table[i] = 1.0 + i;
+    if (!c.table64.upload(table.data(), table.size())) return -1;
+    c.packed = true;
+    c.table_size = 64;
+  } else {
+    if (!c.vals64.allocate(static_cast<size_t>(c.nnz))) return -1;
+    if (hipMemsetAsync(c.vals64.ptr, 0, sizeof(double) * c.nnz, stream()) != hipSuccess) return -1;
+  }
+  c.has64 = true;
+  op.lanes_f64 = lanes;
+  op.flags_f64 = 0;
+  if (!operator_csr_f64(op, x.ptr, y.ptr, false)) return -1;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
+  (void)hipEventRecord(e0, stream());
+  bool ok = true;
+  for (int r = 0; r < reps && ok; ++r) ok = operator_csr_f64(op, x.ptr, y.ptr, false);
+  (void)hipEventRecord(e1, stream());
+  (void)hipEventSynchronize(e1);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!ok) return -1;
+  *ms_out = static_cast<double>(ms) / reps;
+  return hipStreamSynchronize(stream()) == hipSuccess ? 0 : -1;
+}
