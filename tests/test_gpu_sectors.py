@@ -144,3 +144,70 @@ def test_complex_lanczos_mode_matches_dense(engine):
     exact = np.linalg.eigvalsh(M)
     assert np.abs(np.array(res["eigenvalues"][:3]) - exact[:3]).max() <= 1e-10 * np.abs(exact[0])
     assert res["residual_norm"] < 1e-6
+
+
+def test_fermionic_sector_batch_matches_oracle(engine):
+    """Momentum sectors of a 6-site Hubbard ring (3 up + 3 down, fermionic
+    permutation signs, complex characters) as one batch: product and lowest
+    eigenvalues against the oracle's dense sector matrices."""
+    n, up, down = 6, 3, 3
+    expr = models.hubbard(n, models.chain_bonds(n), t=1.0, U=4.0)
+    diag, off = expr.split()
+    perm = models.chain_translation(n)
+    gen = np.concatenate([perm, perm + n]).astype(np.int32)
+    ops, mats = [], []
+    for k in range(n):
+        S = lsamd.Symmetries([lsamd.Symmetry(gen, k)])
+        b = lsamd.DeviceBasis.fermions(n, up, down, S)
+        b.set_norm_tolerance(LS.DEFAULT_NORM_TOLERANCE)
+        b.build()
+        op = lsamd.DeviceOperator(b, expr)
+        op.build_csr(128)
+        ops.append(op)
+        og = O.Group(2 * n, S.elements(), S.characters.real, S.characters.imag, 0)
+        O.lib().ora_group_set_fermionic.argtypes = [O.C.c_void_p, O.C.c_int]
+        O.lib().ora_group_set_norm_eps.argtypes = [O.C.c_void_p, O.C.c_double]
+        assert O.lib().ora_group_set_fermionic(og.h, 2) == 0
+        O.lib().ora_group_set_norm_eps(og.h, LS.DEFAULT_NORM_TOLERANCE)
+        reps = b.states.copy()
+        M = O.matrix_dense(og, 2 * n, 0, reps, O.Terms(diag), O.Terms(off))
+        assert np.abs(M - M.conj().T).max() < 1e-12
+        mats.append(M)
+    batch = lsamd.DeviceSectorBatch(ops)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((batch.number_states, 4)) + 1j * rng.standard_normal((batch.number_states, 4))
+    want = np.concatenate([M @ X[batch.offsets[s]:batch.offsets[s + 1]] for s, M in enumerate(mats)])
+    assert np.abs(batch.matvec(X, 64) - want).max() <= RTOL64 * np.abs(want).max()
+    assert np.abs(batch.matvec(X, 32) - want).max() <= 2e-5 * np.abs(want).max()
+    r = batch.eigensolve(nev=2, block_size=4, max_blocks=8, keep_blocks=2, max_cycles=60, tol=1e-9)
+    assert r["converged"]
+    for s, M in enumerate(mats):
+        exact = np.linalg.eigvalsh(M)[:2]
+        assert np.abs(r["eigenvalues"][s] - exact).max() <= 1e-9 * max(1.0, np.abs(exact).max())
+
+
+def test_chain28_sector_batch_full_size(engine):
+    """All 56 sectors of the 28-site J1-J2 chain (40 116 600 states) in one
+    batch: the exact Majumdar-Ghosh ground states at k = 0 and k = pi, and two
+    complex sectors against single-sector FP64 Lanczos."""
+    n, h = 28, 14
+    expr = models.j1j2_chain(n, 1.0, 0.5)
+    say("chain28 sectors")
+    batch, bases, ops = LS.chain_sector_batch(n, h, expr, keep_operators=True)
+    assert batch.count == 56 and batch.number_states == 40116600
+    sizes = np.diff(batch.offsets)
+    assert sizes.min() == 716014 and sizes.max() == 716728
+    names = [f"{k}{'+' if si == 1 else '-'}" for k, si in batch.sectors]
+    refs = {nm: ops[names.index(nm)] for nm in ("5+", "11-")}
+    del bases, ops
+    batch.autotune(8)
+    r = batch.eigensolve(nev=4, block_size=8, max_blocks=12, keep_blocks=2, max_cycles=40, tol=1e-7)
+    say("cycles", r["cycles"], "fp64", r["fp64_cycles"], "seconds", r["total_seconds"])
+    assert r["converged"] and r["max_residual"] < 1e-6
+    ev = r["eigenvalues"]
+    for nm in ("0+", "14+"):
+        assert abs(ev[names.index(nm)][0] + 1.5 * n) <= 1e-10 * 1.5 * n
+    assert ev[:, 0].min() >= -1.5 * n - 1e-9
+    for nm, op in refs.items():
+        ref = op.lanczos("otf_complex", max_steps=400, k=4, tol=1e-12)
+        assert np.abs(ev[names.index(nm)] - np.array(ref["eigenvalues"][:4])).max() <= 1e-10 * 1.5 * n, nm
