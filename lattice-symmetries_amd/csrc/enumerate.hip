@@ -40,7 +40,14 @@ struct EnumParams {
   // and xB of weight down; candidate r is (rB, rA) = divmod(r, countA).
   int sites, up, down;
   uint64_t rankA0, countA, rankB0, minA, maxA;
+  // candidates from an explicit ascending list instead (list != nullptr)
+  uint64_t const *list;
+  // acceptance filter: group elements tried per candidate; survive_only
+  // accepts a candidate that passes `limit` elements without the norm test
+  int limit, survive_only;
 };
+
+__device__ __forceinline__ uint64_t enum_first(EnumParams const &p, int64_t first);
 
 __device__ __forceinline__ uint64_t unrank_weight(uint64_t const *binom, uint64_t r, int h) {
   uint64_t state = 0;
@@ -69,6 +76,14 @@ __device__ __forceinline__ uint64_t enum_unrank(EnumParams const &p, uint64_t r)
   return unrank_weight(p.binom, r, p.hamming);
 }
 
+__device__ __forceinline__ uint64_t enum_first(EnumParams const &p, int64_t first) {
+  if (p.list != nullptr) return p.list[first];
+  return enum_unrank(p, p.rank0 + static_cast<uint64_t>(first));
+}
+
+// Candidate k of a lane's range given candidate k - 1 (v).
+__device__ __forceinline__ uint64_t enum_step(EnumParams const &p, int64_t first, int64_t k, uint64_t v);
+
 __device__ __forceinline__ uint64_t enum_next(EnumParams const &p, uint64_t v) {
   if (p.hamming == kAllIntegers) return v + 1;
   if (p.hamming == kSpinful) {
@@ -80,8 +95,29 @@ __device__ __forceinline__ uint64_t enum_next(EnumParams const &p, uint64_t v) {
   return gosper_next(v);
 }
 
+// Stabiliser entries a lane keeps while it scans the group out of order; a
+// candidate with more of them is re-scanned in the reference order.
+constexpr int kMaxStabiliser = 4;
+
+// Acceptance filter of the projected enumeration (is_representative with
+// norm > 0, StatesEnumeration.chpl:242-268).  One lane walks its candidates;
+// each loop iteration tests one group element of the lane's current
+// candidate, so lanes refill independently.  Elements are tried in `order`
+// (most rejecting first): the flag "g(x) >= x for every g" does not depend
+// on the order.  The norm sum must follow the reference's reduction order
+// (chunks of kIsRepresentativeLanes, then single elements): the lane records
+// the stabiliser contributions (element, term) and, once every element has
+// passed, replays them in element order -- the skipped terms are exact zeros,
+// so the sum is the reference's bit for bit.  More than kMaxStabiliser
+// contributions send the candidate through a second scan in the reference
+// order.
+__device__ __forceinline__ uint64_t enum_step(EnumParams const &p, int64_t first, int64_t k, uint64_t v) {
+  if (p.list != nullptr) return p.list[first + k];
+  return enum_next(p, v);
+}
+
 template <typename W, int D, int SI, bool LDS>
-__global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView g,
+__global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView g, int const *__restrict__ order,
                                                          uint32_t *__restrict__ bits,
                                                          uint32_t *__restrict__ counts) {
   extern __shared__ uint64_t smem[];
@@ -90,49 +126,72 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
   W *s_masks = reinterpret_cast<W *>(smem);
   double *s_re = reinterpret_cast<double *>(
       smem + (static_cast<size_t>(G) * depth * sizeof(W) + 7) / 8);
+  int *s_order = reinterpret_cast<int *>(s_re + G);
   if constexpr (LDS) {
-    for (int i = threadIdx.x; i < G * depth; i += blockDim.x) s_masks[i] = static_cast<W>(g.masks[i]);
-    for (int i = threadIdx.x; i < G; i += blockDim.x) s_re[i] = g.eig_re[i];
+    // masks and characters staged in scan order, so the scan indexes LDS
+    // directly; s_order maps a scan position back to the element
+    for (int i = threadIdx.x; i < G * depth; i += blockDim.x) {
+      int const pos = i / depth, k = i - pos * depth;
+      s_masks[i] = static_cast<W>(g.masks[static_cast<size_t>(order[pos]) * depth + k]);
+    }
+    for (int i = threadIdx.x; i < G; i += blockDim.x) {
+      s_re[i] = g.eig_re[order[i]];
+      s_order[i] = order[i];
+    }
     __syncthreads();
   }
   int64_t const lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t const first = lane * p.per_lane;
   int64_t remaining = 0;
   if (lane < p.lanes && first < p.total) remaining = min(p.per_lane, p.total - first);
-  W x = remaining > 0 ? static_cast<W>(enum_unrank(p, p.rank0 + static_cast<uint64_t>(first))) : W{0};
+  W x = remaining > 0 ? static_cast<W>(enum_first(p, first)) : W{0};
   uint32_t *my_bits = bits + lane * (p.per_lane / 32);
   W const flip = static_cast<W>(g.flip);
   int const nch4 = (G / kIsRepresentativeLanes) * kIsRepresentativeLanes;
+  int const limit = p.limit;
   int e = 0;
   int64_t pos = 0;
   uint32_t word = 0, count = 0;
   double total = 0.0, chunk = 0.0;
+  bool in_order = false; // second scan, reference order
+  int nrec = 0;
+  bool overflow = false;
+  int rec_e[kMaxStabiliser];
+  double rec_t[kMaxStabiliser];
 
   while (__any(remaining > 0)) {
     if (remaining > 0) {
+      // element of this iteration: scan position e (LDS holds the scan
+      // order), or element e itself in the in-order re-scan
       W y;
       double re;
-      if constexpr (LDS) {
+      if (LDS && !in_order) {
         y = benes_apply_ptr<W, D>(s_masks + static_cast<size_t>(e) * depth, x, g.shifts, depth);
         re = s_re[e];
       } else {
-        y = group_apply<W, D>(g, e, x);
-        re = g.eig_re[e];
+        int const el = in_order ? e : order[e];
+        y = group_apply<W, D>(g, el, x);
+        re = g.eig_re[el];
       }
       bool ok = y >= x;
+      bool stab = y == x;
       double t = static_cast<double>(y == x) * re;
       if constexpr (SI == kFermionic) {
-        if (y == x) t = re * fermion_sign(g, e, static_cast<uint64_t>(x));
+        if (y == x) {
+          int const el = in_order ? e : (LDS ? s_order[e] : order[e]);
+          t = re * fermion_sign(g, el, static_cast<uint64_t>(x));
+        }
       } else if constexpr (SI != 0) {
         W const yf = static_cast<W>(y ^ flip);
         ok = ok && (yf >= x);
+        stab = stab || (yf == x);
         double const a = static_cast<double>(y == x), b = static_cast<double>(yf == x);
         t = (SI == 1 ? (a + b) : (a - b)) * re;
       }
       bool done = false, accept = false;
       if (!ok) {
         done = true;
-      } else {
+      } else if (in_order) {
         if (e < nch4) {
           chunk = chunk + t;
           if ((e & 3) == 3) {
@@ -147,6 +206,74 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
           done = true;
           accept = total > g.norm_eps;
         }
+      } else {
+        if (stab) {
+          if (nrec < kMaxStabiliser) {
+            int const el = LDS ? s_order[e] : order[e];
+#pragma unroll
+            for (int r = 0; r < kMaxStabiliser; ++r)
+              if (r == nrec) {
+                rec_e[r] = el;
+                rec_t[r] = t;
+              }
+            ++nrec;
+          } else {
+            overflow = true;
+          }
+        }
+        ++e;
+        if (p.survive_only && e == limit) {
+          done = true; // passed the first `limit` elements: a survivor
+          accept = true;
+        } else if (e == G) {
+          if (overflow) {
+            in_order = true; // every element passed; recompute the norm in order
+            e = 0;
+            total = 0.0;
+            chunk = 0.0;
+          } else {
+            // replay the recorded terms in element order with the
+            // reference's chunking
+#pragma unroll
+            for (int a = 1; a < kMaxStabiliser; ++a)
+#pragma unroll
+              for (int b = kMaxStabiliser - 1; b >= a; --b)
+                if (b < nrec && rec_e[b - 1] > rec_e[b]) {
+                  int const te = rec_e[b - 1];
+                  rec_e[b - 1] = rec_e[b];
+                  rec_e[b] = te;
+                  double const tt = rec_t[b - 1];
+                  rec_t[b - 1] = rec_t[b];
+                  rec_t[b] = tt;
+                }
+            total = 0.0;
+            chunk = 0.0;
+            int cur = -1;
+#pragma unroll
+            for (int r = 0; r < kMaxStabiliser; ++r) {
+              if (r < nrec) {
+                int const er = rec_e[r];
+                if (er < nch4) {
+                  int const c = er >> 2;
+                  if (c != cur) {
+                    total = total + chunk;
+                    chunk = 0.0;
+                    cur = c;
+                  }
+                  chunk = chunk + rec_t[r];
+                } else {
+                  total = total + chunk;
+                  chunk = 0.0;
+                  cur = -1;
+                  total = total + rec_t[r];
+                }
+              }
+            }
+            total = total + chunk;
+            done = true;
+            accept = total > g.norm_eps;
+          }
+        }
       }
       if (done) {
         if (accept) {
@@ -159,10 +286,13 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
           my_bits[(pos - 1) >> 5] = word;
           word = 0;
         }
-        if (remaining > 0) x = static_cast<W>(enum_next(p, static_cast<uint64_t>(x)));
+        if (remaining > 0) x = static_cast<W>(enum_step(p, first, pos, static_cast<uint64_t>(x)));
         e = 0;
         total = 0.0;
         chunk = 0.0;
+        in_order = false;
+        nrec = 0;
+        overflow = false;
       }
     }
   }
@@ -181,14 +311,14 @@ __global__ __launch_bounds__(256) void enum_write_kernel(EnumParams p,
   int64_t const first = lane * p.per_lane;
   if (first >= p.total) return;
   int64_t const n = min(p.per_lane, p.total - first);
-  uint64_t x = enum_unrank(p, p.rank0 + static_cast<uint64_t>(first));
+  uint64_t x = enum_first(p, first);
   uint64_t o = offsets != nullptr ? offsets[lane] : static_cast<uint64_t>(first);
   uint32_t const *my_bits = bits != nullptr ? bits + lane * (p.per_lane / 32) : nullptr;
   uint32_t word = 0xffffffffu;
   for (int64_t k = 0; k < n; ++k) {
     if (my_bits != nullptr && (k & 31) == 0) word = my_bits[k >> 5];
     if ((word >> (k & 31)) & 1u) out[o++] = x;
-    if (k + 1 < n) x = enum_next(p, x);
+    if (k + 1 < n) x = enum_step(p, first, k + 1, x);
   }
 }
 
@@ -292,6 +422,10 @@ bool exclusive_scan(uint32_t const *d_counts, int64_t n, uint64_t *d_offsets, ui
 namespace {
 // Candidates per lane: enough lanes to fill the chip several times over
 // (256 CUs x 32 waves x 64 lanes ~ 0.5 M), at least 64 per lane.
+// Group elements a candidate must pass in the first filter pass before it
+// goes to the full test.
+constexpr int kSurvivorElements = 24;
+
 int64_t lanes_for(int64_t total, int64_t &per_lane) {
   int64_t const target_lanes = int64_t{1} << 21;
   per_lane = (total + target_lanes - 1) / target_lanes;
@@ -370,32 +504,71 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
   }
 
   GroupData const &g = *spec.group;
-  DeviceArray<uint32_t> bits, counts;
-  DeviceArray<uint64_t> offsets;
-  if (!bits.allocate(static_cast<size_t>(p.lanes * (p.per_lane / 32)))) return false;
-  if (!counts.allocate(static_cast<size_t>(p.lanes))) return false;
-  if (!offsets.allocate(static_cast<size_t>(p.lanes))) return false;
   GroupView const v = g.view();
-  timer_begin(kTimerEnumerate);
-  bool ok = dispatch_group(g, [&]<typename W, int D, int SI>() -> bool {
-    int const depth = D > 0 ? D : g.depth;
-    size_t const smem = ((static_cast<size_t>(g.size) * depth * sizeof(W) + 7) / 8) * 8 +
-                        static_cast<size_t>(g.size) * sizeof(double);
-    if (smem <= 64 * 1024) {
-      enum_count_kernel<W, D, SI, true><<<blocks, 256, smem, stream()>>>(p, v, bits.ptr, counts.ptr);
-    } else {
-      enum_count_kernel<W, D, SI, false><<<blocks, 256, 0, stream()>>>(p, v, bits.ptr, counts.ptr);
+  std::vector<int> const h_order =
+      group_rejection_order(g, p.hamming >= 0 ? p.hamming : (p.hamming == kSpinful ? p.up + p.down : -1),
+                            g.number_bits);
+  DeviceArray<int> d_order;
+  if (!d_order.upload(h_order.data(), h_order.size())) return false;
+  // One filter pass over candidates q (ranked or listed): bits, counts,
+  // scan, then the accepted ones written to `dst` in order.
+  auto filter = [&](EnumParams const &q, DeviceArray<uint64_t> &dst, uint64_t &kept) -> bool {
+    int const nblocks = grid_for(q.lanes, 256);
+    DeviceArray<uint32_t> bits, counts;
+    DeviceArray<uint64_t> offsets;
+    if (!bits.allocate(static_cast<size_t>(q.lanes * (q.per_lane / 32)))) return false;
+    if (!counts.allocate(static_cast<size_t>(q.lanes))) return false;
+    if (!offsets.allocate(static_cast<size_t>(q.lanes))) return false;
+    bool ok = dispatch_group(g, [&]<typename W, int D, int SI>() -> bool {
+      int const depth = D > 0 ? D : g.depth;
+      size_t const smem = ((static_cast<size_t>(g.size) * depth * sizeof(W) + 7) / 8) * 8 +
+                          static_cast<size_t>(g.size) * (sizeof(double) + sizeof(int));
+      if (smem <= 64 * 1024) {
+        enum_count_kernel<W, D, SI, true><<<nblocks, 256, smem, stream()>>>(q, v, d_order.ptr, bits.ptr, counts.ptr);
+      } else {
+        enum_count_kernel<W, D, SI, false><<<nblocks, 256, 0, stream()>>>(q, v, d_order.ptr, bits.ptr, counts.ptr);
+      }
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    });
+    if (!ok) return false;
+    kept = 0;
+    if (!exclusive_scan(counts.ptr, q.lanes, offsets.ptr, kept)) return false;
+    if (!dst.allocate(static_cast<size_t>(kept))) return false;
+    if (kept > 0) {
+      enum_write_kernel<<<nblocks, 256, 0, stream()>>>(q, bits.ptr, offsets.ptr, dst.ptr);
+      LSAMD_HIP(hipGetLastError());
     }
-    LSAMD_HIP(hipGetLastError());
     return true;
-  });
-  if (!ok) return false;
+  };
+  timer_begin(kTimerEnumerate);
   uint64_t total = 0;
-  if (!exclusive_scan(counts.ptr, p.lanes, offsets.ptr, total)) return false;
-  if (!out.allocate(static_cast<size_t>(total))) return false;
-  if (total > 0) {
-    enum_write_kernel<<<blocks, 256, 0, stream()>>>(p, bits.ptr, offsets.ptr, out.ptr);
-    LSAMD_HIP(hipGetLastError());
+  if (g.size <= kSurvivorElements) {
+    p.limit = g.size;
+    p.survive_only = 0;
+    if (!filter(p, out, total)) return false;
+  } else {
+    // Pass 1: every candidate against the first elements of the scan order
+    // (nearly all are rejected by the first one or two).  Pass 2: the
+    // survivors, now a short ascending list, against the whole group, spread
+    // evenly over the lanes.
+    DeviceArray<uint64_t> survivors;
+    uint64_t nsurv = 0;
+    p.limit = kSurvivorElements;
+    p.survive_only = 1;
+    if (!filter(p, survivors, nsurv)) return false;
+    EnumParams q = p;
+    q.list = survivors.ptr;
+    q.rank0 = 0;
+    q.total = static_cast<int64_t>(nsurv);
+    q.limit = g.size;
+    q.survive_only = 0;
+    q.lanes = lanes_for(q.total, q.per_lane);
+    if (nsurv > 0) {
+      if (!filter(q, out, total)) return false;
+    } else {
+      out.reset();
+    }
   }
   timer_end(kTimerEnumerate);
   LSAMD_HIP(hipStreamSynchronize(stream()));

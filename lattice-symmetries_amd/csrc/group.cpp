@@ -229,6 +229,82 @@ GroupData *group_data_create(ls_hs_permutation_group const *g, int spin_inversio
   return d.release();
 }
 
+// Order in which the enumeration filter tries the group elements: the
+// predicate "g(x) >= x for every g" does not depend on the order, so the
+// elements that reject most candidates go first.  Greedy over a seeded
+// sample of candidates: each step takes the element that rejects the most
+// sample states not yet rejected; the rest follow by rejection count.
+std::vector<int> group_rejection_order(GroupData const &g, int hamming, int number_bits) {
+  int const G = g.size, depth = g.depth;
+  std::vector<int> order(G);
+  for (int e = 0; e < G; ++e) order[e] = e;
+  if (G <= 2 || number_bits <= 0 || number_bits > 64) return order;
+  int const samples = 1024;
+  std::vector<uint64_t> xs(samples);
+  uint64_t z = 0x243f6a8885a308d3ull;
+  auto next = [&]() {
+    z += 0x9e3779b97f4a7c15ull;
+    uint64_t r = z;
+    r = (r ^ (r >> 30)) * 0xbf58476d1ce4e5b9ull;
+    r = (r ^ (r >> 27)) * 0x94d049bb133111ebull;
+    return r ^ (r >> 31);
+  };
+  uint64_t const all = number_bits == 64 ? ~uint64_t{0} : ((uint64_t{1} << number_bits) - 1);
+  for (auto &x : xs) {
+    if (hamming >= 0 && hamming <= number_bits) {
+      x = 0;
+      int placed = 0;
+      while (placed < hamming) {
+        int const b = static_cast<int>(next() % static_cast<uint64_t>(number_bits));
+        if (!((x >> b) & 1)) {
+          x |= uint64_t{1} << b;
+          ++placed;
+        }
+      }
+    } else {
+      x = next() & all;
+    }
+  }
+  std::vector<std::vector<uint8_t>> rej(G, std::vector<uint8_t>(samples));
+  std::vector<int> count(G, 0);
+  for (int e = 0; e < G; ++e) {
+    std::vector<uint64_t> m(g.h_masks.begin() + static_cast<size_t>(e) * depth,
+                            g.h_masks.begin() + static_cast<size_t>(e + 1) * depth);
+    for (int s = 0; s < samples; ++s) {
+      uint64_t const y = apply_network(m, g.h_shifts, xs[s]);
+      bool r = y < xs[s];
+      if (g.spin_inversion != 0) r = r || ((y ^ g.flip_mask) < xs[s]);
+      rej[e][s] = r ? 1 : 0;
+      count[e] += r ? 1 : 0;
+    }
+  }
+  std::vector<uint8_t> alive(samples, 1), used(G, 0);
+  std::vector<int> out;
+  for (int step = 0; step < std::min(G, 32); ++step) {
+    int best = -1, best_gain = 0;
+    for (int e = 0; e < G; ++e) {
+      if (used[e]) continue;
+      int gain = 0;
+      for (int s = 0; s < samples; ++s) gain += alive[s] & rej[e][s];
+      if (gain > best_gain) {
+        best_gain = gain;
+        best = e;
+      }
+    }
+    if (best < 0) break;
+    used[best] = 1;
+    out.push_back(best);
+    for (int s = 0; s < samples; ++s)
+      if (rej[best][s]) alive[s] = 0;
+  }
+  std::vector<int> rest;
+  for (int e = 0; e < G; ++e)
+    if (!used[e]) rest.push_back(e);
+  std::stable_sort(rest.begin(), rest.end(), [&](int a, int b) { return count[a] > count[b]; });
+  out.insert(out.end(), rest.begin(), rest.end());
+  return out;
+}
+
 bool group_make_fermionic(GroupData &g, int species) {
   if (species != 1 && species != 2) return fail("fermion species must be 1 or 2");
   if (g.spin_inversion != 0) return fail("fermionic bases have no spin inversion");

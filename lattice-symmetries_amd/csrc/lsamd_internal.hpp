@@ -180,6 +180,9 @@ struct GroupData {
 // Marks a group as fermionic (species 1 or 2) and uploads the orbital
 // images; spin inversion must be off.
 bool group_make_fermionic(GroupData &g, int species);
+// Element order for the enumeration filter (most rejecting first), from a
+// seeded sample of candidates of the given Hamming weight (-1: any).
+std::vector<int> group_rejection_order(GroupData const &g, int hamming, int number_bits);
 
 // Private data behind ls_hs_create_state_index_binary_search_kernel_data.
 struct IndexData {
