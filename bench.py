@@ -129,7 +129,7 @@ def sectors_bench(args):
     traffic, traffic_src = profiled_traffic(f"chain{n}_sectors", sector_kernel)
     value = 8.0 * nnz * args.steps / t_steps
     del dx, dy
-    r = batch.eigensolve(nev=4, block_size=8, max_blocks=args.sector_blocks, keep_blocks=2,
+    r = batch.eigensolve(nev=4, block_size=8, max_blocks=args.sector_blocks, keep_blocks=args.sector_keep,
                          max_cycles=args.sector_cycles, tol=1e-7, switch_tol=1e-5, seed=1)
     log(f"eigensolve: {r['cycles']} cycles ({r['fp64_cycles']} FP64), {r['products']} products, "
         f"{r['total_seconds']:.1f} s, max residual {r['max_residual']:.2e}")
@@ -236,6 +236,7 @@ def main():
     ap.add_argument("--chain", type=int, default=28)
     ap.add_argument("--sector-blocks", type=int, default=12)
     ap.add_argument("--sector-cycles", type=int, default=40)
+    ap.add_argument("--sector-keep", type=int, default=2)
     ap.add_argument("--reference-sectors", default="0+,14+,7-,13+")
     ap.add_argument("--reference-steps", type=int, default=400)
     args = ap.parse_args()
