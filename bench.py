@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--chain", type=int, default=28)
     ap.add_argument("--sector-blocks", type=int, default=12)
     ap.add_argument("--sector-cycles", type=int, default=40)
-    ap.add_argument("--sector-keep", type=int, default=2)
+    ap.add_argument("--sector-keep", type=int, default=3)
     ap.add_argument("--reference-sectors", default="0+,14+,7-,13+")
     ap.add_argument("--reference-steps", type=int, default=400)
     args = ap.parse_args()
