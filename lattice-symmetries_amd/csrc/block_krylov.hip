@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kThreads) void seg_gram_kernel(double const *__rest
   constexpr int V = B * B;
   __shared__ double s[kThreads / 64][64];
   int const c = blockIdx.x;
-  int64_t const r0 = chunk_rows[c], r1 = chunk_rows[c + 1];
+  int64_t const r0 = chunk_rows[2 * c], r1 = chunk_rows[2 * c + 1];
   int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = 0; i < nblk; ++i) {
     double const *__restrict__ Ai = A + static_cast<int64_t>(i) * a_stride;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void seg_combine_kernel(double *out, doub
   extern __shared__ double sc[];
   int const c = blockIdx.x;
   int const s = chunk_seg[c];
-  int64_t const r0 = chunk_rows[c], r1 = chunk_rows[c + 1];
+  int64_t const r0 = chunk_rows[2 * c], r1 = chunk_rows[2 * c + 1];
   double const *__restrict__ cs = coef + static_cast<int64_t>(s) * nblk * V;
   for (int t = threadIdx.x; t < nblk * V; t += kThreads) sc[t] = cs[t];
   __syncthreads();
@@ -185,28 +185,31 @@ template <typename F> bool dispatch_block(int b, F &&f) {
   }
 }
 
-// Chunks of the rows, per segment.
+// Chunks of the rows, per segment.  Only the listed segments get chunks
+// (numbered 0 .. nseg - 1 in list order); rows of the others are never read
+// or written by the block algebra.
 struct Segments {
   int nseg = 0;
   int64_t rows = 0;
   int chunks = 0;
-  std::vector<int64_t> row_begin; // first row of every segment, nseg + 1
-  DeviceArray<int64_t> d_chunk_rows;
+  DeviceArray<int64_t> d_chunk_rows; // chunk c covers rows [chunk_rows[2c], chunk_rows[2c + 1])
   DeviceArray<int> d_chunk_seg, d_seg_chunk;
-  bool init(std::vector<int64_t> const &seg_rows) {
-    nseg = static_cast<int>(seg_rows.size()) - 1;
-    row_begin = seg_rows;
-    rows = row_begin[nseg];
-    std::vector<int64_t> cr{0};
+  bool init(std::vector<int64_t> const &seg_rows, std::vector<int> const &listed) {
+    nseg = static_cast<int>(listed.size());
+    rows = seg_rows.back();
+    std::vector<int64_t> cr;
     std::vector<int> cs, sc{0};
-    for (int s = 0; s < nseg; ++s) {
-      for (int64_t r = row_begin[s]; r < row_begin[s + 1]; r += kChunkRows) {
-        cr.push_back(std::min(r + kChunkRows, row_begin[s + 1]));
-        cs.push_back(s);
+    for (int a = 0; a < nseg; ++a) {
+      int64_t const r0 = seg_rows[listed[a]], r1 = seg_rows[listed[a] + 1];
+      for (int64_t r = r0; r < r1; r += kChunkRows) {
+        cr.push_back(r);
+        cr.push_back(std::min(r + kChunkRows, r1));
+        cs.push_back(a);
       }
       sc.push_back(static_cast<int>(cs.size()));
     }
     chunks = static_cast<int>(cs.size());
+    if (cr.empty()) cr.assign(2, 0);
     return d_chunk_rows.upload(cr.data(), cr.size()) && d_chunk_seg.upload(cs.data(), std::max<size_t>(1, cs.size())) &&
            d_seg_chunk.upload(sc.data(), sc.size());
   }
@@ -277,10 +280,10 @@ struct BlockAlgebra {
   DeviceArray<double> partial, d_out, d_coef;
   std::vector<double> h_coef;
   int max_nblk = 0;
-  bool init(std::vector<int64_t> const &offsets, int b, int nblk_max) {
+  bool init(std::vector<int64_t> const &offsets, std::vector<int> const &listed, int b, int nblk_max) {
     B = b;
     max_nblk = nblk_max;
-    if (!seg.init(offsets)) return false;
+    if (!seg.init(offsets, listed)) return false;
     size_t const V = static_cast<size_t>(B) * B;
     return partial.allocate(std::max<size_t>(1, static_cast<size_t>(seg.chunks) * nblk_max * V)) &&
            d_out.allocate(static_cast<size_t>(seg.nseg) * nblk_max * V) &&
@@ -354,32 +357,53 @@ bool block_column(double const *X, int b, int col, double *out, int64_t rows) {
   return true;
 }
 
-bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, BlockProduct const &product,
-                        BlockKrylovOptions const &o, BlockKrylovResult &res, double *d_ritz_block) {
-  auto const t_start = Clock::now();
+bool check_block_krylov_options(BlockKrylovOptions const &o, int multiplicity) {
   int const B = o.block;
   if (B != 1 && B != 2 && B != 4 && B != 8) return fail("block size must be 1, 2, 4 or 8");
-  int const want = multiplicity * o.nev; // Ritz values per segment
-  if (o.nev < 1 || want > B) return fail("nev (times 2 for complex sectors) must not exceed the block size");
+  if (o.nev < 1 || multiplicity * o.nev > B)
+    return fail("nev (times 2 for complex sectors) must not exceed the block size");
   if (o.keep < 1 || o.keep >= o.max_blocks - 1) return fail("keep must be in [1, max_blocks - 2]");
   if (o.max_cycles < 1) return fail("max_cycles must be positive");
-  int const ns = static_cast<int>(seg_rows.size()) - 1;
+  return true;
+}
+
+bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const &active, int multiplicity,
+                        BlockProduct const &product, BlockKrylovOptions const &o, BlockKrylovResult &res,
+                        double *d_ritz_block) {
+  auto const t_start = Clock::now();
+  if (!check_block_krylov_options(o, multiplicity)) return false;
+  int const B = o.block;
+  int const want = multiplicity * o.nev; // Ritz values per segment
+  int const nseg_all = static_cast<int>(seg_rows.size()) - 1;
+  if (nseg_all < 1) return fail("no segments");
+  if (!active.empty() && static_cast<int>(active.size()) != nseg_all) return fail("internal: bad segment mask");
+  // the segments solved here, renumbered 0 .. ns - 1
+  std::vector<int> listed;
+  for (int s = 0; s < nseg_all; ++s)
+    if (active.empty() || active[s]) listed.push_back(s);
+  int const ns = static_cast<int>(listed.size());
   if (ns < 1) return fail("no segments");
   int const m = o.max_blocks;
   int const V = B * B;
   int64_t min_rows = INT64_MAX;
-  for (int s = 0; s < ns; ++s) min_rows = std::min(min_rows, seg_rows[s + 1] - seg_rows[s]);
+  for (int s : listed) min_rows = std::min(min_rows, seg_rows[s + 1] - seg_rows[s]);
   if (min_rows < static_cast<int64_t>(m + 1) * B)
     return fail("a segment is too small for (max_blocks + 1) * block_size Krylov vectors (" +
                 std::to_string(min_rows) + " rows)");
   BlockAlgebra alg;
-  if (!alg.init(seg_rows, B, m)) return false;
+  if (!alg.init(seg_rows, listed, B, m)) return false;
   int64_t const rows = alg.seg.rows;
   int64_t const blk = rows * B; // doubles per block
   DeviceArray<double> Vb, W, Y;
   if (!Vb.allocate(static_cast<size_t>(blk) * m) || !W.allocate(static_cast<size_t>(blk)) ||
       !Y.allocate(static_cast<size_t>(blk) * o.keep))
     return false;
+  if (ns < nseg_all) {
+    // rows outside the solved segments are only ever read by the product,
+    // which keeps them inside their own (block-diagonal) segment
+    LSAMD_HIP(hipMemsetAsync(Vb.ptr, 0, sizeof(double) * blk * m, stream()));
+    LSAMD_HIP(hipMemsetAsync(Y.ptr, 0, sizeof(double) * blk * o.keep, stream()));
+  }
   auto Vj = [&](int j) { return Vb.ptr + static_cast<int64_t>(j) * blk; };
 
   double prod_s = 0.0;
@@ -548,8 +572,8 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, 
   for (size_t t = 0; t < Th.size(); ++t) negTh[t] = -Th[t];
   if (!alg.combine(W.ptr, 1.0, Y.ptr, blk, 1, negTh)) return false; // A Y - Y (Y^T A Y)
   if (!alg.gram(W.ptr, blk, 1, W.ptr, Gr)) return false;
-  res.evals.assign(static_cast<size_t>(ns) * o.nev, 0.0);
-  res.residuals.assign(static_cast<size_t>(ns) * o.nev, 0.0);
+  res.evals.assign(static_cast<size_t>(nseg_all) * o.nev, NAN);
+  res.residuals.assign(static_cast<size_t>(nseg_all) * o.nev, NAN);
   double rmax = 0.0;
   for (int s = 0; s < ns; ++s)
     for (int e = 0; e < o.nev; ++e) {
@@ -561,8 +585,8 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, 
       }
       lam /= multiplicity;
       double const rel = std::sqrt(std::max(0.0, r2)) / std::max(1.0, std::fabs(lam));
-      res.evals[static_cast<size_t>(s) * o.nev + e] = lam;
-      res.residuals[static_cast<size_t>(s) * o.nev + e] = rel;
+      res.evals[static_cast<size_t>(listed[s]) * o.nev + e] = lam;
+      res.residuals[static_cast<size_t>(listed[s]) * o.nev + e] = rel;
       rmax = std::max(rmax, rel);
     }
   res.first_rayleigh = Th[0];

@@ -469,7 +469,7 @@ bool operator_block_solve(OperatorCore &core, int mode, BlockKrylovOptions const
     return operator_csr_f64_block(core, o.block, x, y);
   };
   std::vector<int64_t> const seg{0, n};
-  return block_krylov_solve(seg, 1, prod, o, r, d_ritz_block);
+  return block_krylov_solve(seg, {}, 1, prod, o, r, d_ritz_block);
 }
 
 void fill_info(ls_amd_eig_info *info, BlockKrylovResult const &r) {

@@ -126,9 +126,13 @@ using BlockProduct = std::function<bool(bool fp32, double const *x, double *y)>;
 
 // Each eigenvalue appears `multiplicity` times in the segments' spectra (2 for
 // the real embedding of complex sectors); d_ritz_block (may be null) receives
-// the lowest Ritz block.
-bool block_krylov_solve(std::vector<int64_t> const &seg_rows, int multiplicity, BlockProduct const &product,
-                        BlockKrylovOptions const &o, BlockKrylovResult &res, double *d_ritz_block);
+// the lowest Ritz block.  Segment s spans rows [seg_rows[s], seg_rows[s + 1]);
+// only segments with active[s] != 0 are solved (all of them if `active` is
+// empty), the others' results are NaN and their rows are left to the product.
+bool check_block_krylov_options(BlockKrylovOptions const &o, int multiplicity);
+bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const &active, int multiplicity,
+                        BlockProduct const &product, BlockKrylovOptions const &o, BlockKrylovResult &res,
+                        double *d_ritz_block);
 bool block_to_f32(double const *x, float *y, int64_t count);
 
 // Dense symmetric eigen-decomposition (dense_eigh.cpp): A is n x n row-major;

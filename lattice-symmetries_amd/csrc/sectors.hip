@@ -21,6 +21,7 @@
 // python/lattice_symmetries/__init__.py:591-752) with the real-valued
 // product of DistributedMatrixVector.chpl:70-140; there is no batched form.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -364,25 +365,125 @@ bool sector_autotune(ls_amd_sector_batch &b, int k) {
 // Block Lanczos over every sector at once
 // ---------------------------------------------------------------------------
 
+// Largest sector diagonalised densely on the host (its real embedding has
+// twice as many rows; about 8 (2 d)^3 flops on one host thread).
+constexpr int64_t kDenseSectorMax = 512;
+
+// Lowest nev eigenvalues of sector s by dense diagonalisation of the real
+// embedding [[A, -B], [B, A]] of its Hermitian matrix M = A + iB (every
+// eigenvalue of M appears twice), read from the FP64 CSR.  Eigenvalues past
+// the sector's dimension are NaN; residuals are || E z - lambda z || of the
+// embedding's eigenvectors, relative to max(1, |lambda|).
+bool dense_sector(ls_amd_sector_batch const &bt, int s, int nev, double *evals, double *resid) {
+  int64_t const r0 = bt.offsets[s], d = bt.offsets[s + 1] - r0;
+  std::fill(evals, evals + nev, NAN);
+  std::fill(resid, resid + nev, NAN);
+  if (d == 0) return true;
+  Csr const &c = bt.csr;
+  std::vector<int64_t> rp(static_cast<size_t>(d + 1));
+  std::vector<double> diag(static_cast<size_t>(d));
+  LSAMD_HIP(hipMemcpyAsync(rp.data(), c.row_ptr.ptr + r0, sizeof(int64_t) * (d + 1), hipMemcpyDeviceToHost, stream()));
+  LSAMD_HIP(hipMemcpyAsync(diag.data(), c.diag64.ptr + r0, sizeof(double) * d, hipMemcpyDeviceToHost, stream()));
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  int64_t const k0 = rp[0], nz = rp[d] - k0;
+  std::vector<uint32_t> cols(static_cast<size_t>(std::max<int64_t>(nz, 1)));
+  std::vector<double> vals(static_cast<size_t>(2 * std::max<int64_t>(nz, 1)));
+  if (nz > 0) {
+    LSAMD_HIP(hipMemcpyAsync(cols.data(), c.cols.ptr + k0, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost, stream()));
+    LSAMD_HIP(hipMemcpyAsync(vals.data(), c.vals64.ptr + 2 * k0, sizeof(double) * 2 * nz, hipMemcpyDeviceToHost,
+                             stream()));
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+  }
+  int const n2 = static_cast<int>(2 * d);
+  std::vector<double> E(static_cast<size_t>(n2) * n2, 0.0);
+  auto at = [&](int64_t i, int64_t j) -> double & { return E[static_cast<size_t>(i) * n2 + j]; };
+  for (int64_t i = 0; i < d; ++i) {
+    at(i, i) += diag[i];
+    at(d + i, d + i) += diag[i];
+    for (int64_t k = rp[i] - k0; k < rp[i + 1] - k0; ++k) {
+      int64_t const j = static_cast<int64_t>(cols[k]) - r0;
+      if (j < 0 || j >= d) return fail("sector " + std::to_string(s) + ": a matrix element leaves the sector");
+      double const a = vals[2 * k], b = vals[2 * k + 1];
+      at(i, j) += a;
+      at(d + i, d + j) += a;
+      at(i, d + j) -= b;
+      at(d + i, j) += b;
+    }
+  }
+  std::vector<double> Es(E.size());
+  for (int i = 0; i < n2; ++i)
+    for (int j = 0; j < n2; ++j) Es[static_cast<size_t>(i) * n2 + j] = 0.5 * (at(i, j) + at(j, i));
+  std::vector<double> w, Z;
+  if (!symmetric_eigh(Es, n2, w, Z)) return fail("sector " + std::to_string(s) + ": dense eigensolver did not converge");
+  for (int e = 0; e < nev && e < d; ++e) {
+    double const lam = 0.5 * (w[2 * e] + w[2 * e + 1]);
+    double r2 = 0.0;
+    for (int i = 0; i < n2; ++i) {
+      double acc = -lam * Z[static_cast<size_t>(i) * n2 + 2 * e];
+      for (int j = 0; j < n2; ++j) acc += at(i, j) * Z[static_cast<size_t>(j) * n2 + 2 * e];
+      r2 += acc * acc;
+    }
+    evals[e] = lam;
+    resid[e] = std::sqrt(r2) / std::max(1.0, std::fabs(lam));
+  }
+  return true;
+}
+
 // The sectors' rows in the real embedding are [2 offsets[s], 2 offsets[s+1]);
-// each eigenvalue of a sector appears twice there (multiplicity 2).
+// each eigenvalue of a sector appears twice there (multiplicity 2).  Sectors
+// with fewer states than the Krylov space, (max_blocks + 1) * block, are
+// diagonalised densely instead and left out of the block Lanczos process.
 bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &o, BlockKrylovResult &res) {
+  auto const t_start = std::chrono::steady_clock::now();
+  int const ns = bt.count;
+  int64_t const krylov_min = static_cast<int64_t>(o.max_blocks + 1) * o.block;
   std::vector<int64_t> seg_rows(bt.offsets.size());
   for (size_t s = 0; s < seg_rows.size(); ++s) seg_rows[s] = 2 * bt.offsets[s];
-  int64_t const count = 2 * bt.csr.rows * o.block;
-  DeviceArray<float> X32;
-  if (!X32.allocate(static_cast<size_t>(std::max<int64_t>(count, 1)))) return false;
-  int const B = o.block;
-  BlockProduct prod = [&](bool fp32, double const *x, double *y) -> bool {
-    if (fp32) {
-      if (!block_to_f32(x, X32.ptr, count)) return false;
-      return graph_product(bt, 2, B, X32.ptr, y);
+  std::vector<char> active(ns, 1);
+  int dense = 0;
+  for (int s = 0; s < ns; ++s) {
+    int64_t const d = bt.offsets[s + 1] - bt.offsets[s];
+    if (d < krylov_min) {
+      if (d > kDenseSectorMax)
+        return fail("sector " + std::to_string(s) + " (" + std::to_string(d) +
+                    " states) is too small for the Krylov space and too large to diagonalise densely");
+      active[s] = 0;
+      ++dense;
     }
-    if (!launch_product(bt, 0, B, x, y, bt.lanes, bt.blocks, bt.xcd_aware)) return false;
-    LSAMD_HIP(hipGetLastError());
-    return true;
-  };
-  return block_krylov_solve(seg_rows, 2, prod, o, res, nullptr);
+  }
+  if (dense < ns) {
+    int64_t const count = 2 * bt.csr.rows * o.block;
+    DeviceArray<float> X32;
+    if (!X32.allocate(static_cast<size_t>(std::max<int64_t>(count, 1)))) return false;
+    int const B = o.block;
+    BlockProduct prod = [&](bool fp32, double const *x, double *y) -> bool {
+      if (fp32) {
+        if (!block_to_f32(x, X32.ptr, count)) return false;
+        return graph_product(bt, 2, B, X32.ptr, y);
+      }
+      if (!launch_product(bt, 0, B, x, y, bt.lanes, bt.blocks, bt.xcd_aware)) return false;
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    };
+    if (!block_krylov_solve(seg_rows, active, 2, prod, o, res, nullptr)) return false;
+  } else {
+    // nothing for the Krylov process; the options are still checked alike
+    if (!check_block_krylov_options(o, 2)) return false;
+    res = BlockKrylovResult{};
+    res.converged = true;
+    res.evals.assign(static_cast<size_t>(ns) * o.nev, NAN);
+    res.residuals.assign(static_cast<size_t>(ns) * o.nev, NAN);
+  }
+  for (int s = 0; s < ns; ++s) {
+    if (active[s]) continue;
+    double *ev = res.evals.data() + static_cast<size_t>(s) * o.nev;
+    double *rr = res.residuals.data() + static_cast<size_t>(s) * o.nev;
+    if (!dense_sector(bt, s, o.nev, ev, rr)) return false;
+    for (int e = 0; e < o.nev; ++e)
+      if (std::isfinite(rr[e])) res.max_residual = std::max(res.max_residual, rr[e]);
+  }
+  res.total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  return true;
 }
 
 } // namespace lsamd

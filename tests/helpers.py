@@ -185,3 +185,32 @@ def burnside_by_cycles(n, h, perms, chars, spin_inversion):
                 total += spin_inversion * np.conj(c) * (2 ** len(cyc))
     order = len(perms) * (2 if spin_inversion else 1)
     return total / order
+
+
+def chain_sector_matrices(n, h, expr, sector_list, norm_eps):
+    """Oracle representatives and dense projected matrices of chain sectors
+    (momentum k, spin inversion +1 / -1 / None), with the given zero
+    threshold for stabiliser sums."""
+    O.lib().ora_group_set_norm_eps.argtypes = [O.C.c_void_p, O.C.c_double]
+    diag, off = expr.split()
+    out = []
+    for k, si in sector_list:
+        perms, num, den, re, im = O.group_closure(n, [models.chain_translation(n)], [k])
+        g = O.Group(n, perms, re, im, si or 0)
+        O.lib().ora_group_set_norm_eps(g.h, norm_eps)
+        lo = (1 << h) - 1
+        hi = lo << (n - h - 1) if si else lo << (n - h)
+        reps = g.enumerate(lo, hi)
+        M = O.matrix_dense(g, n, si or 0, reps, O.Terms(diag), O.Terms(off))
+        out.append((reps, M))
+    return out
+
+
+def chain_sector_spectrum(n, h, expr, k, si):
+    """Spectrum of a chain sector from the full weight-h Hamiltonian projected
+    onto explicitly symmetrised vectors (numpy only)."""
+    perms, num, den, re, im = O.group_closure(n, [models.chain_translation(n)], [k])
+    states = weight_states(n, h)
+    full = term_matrix(n, sum(expr.split(), []))
+    M = sector_matrix_numpy(n, perms, re + 1j * im, si or 0, full[np.ix_(states, states)], states)
+    return np.linalg.eigvalsh(M) if M.size else np.zeros(0)

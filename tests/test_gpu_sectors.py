@@ -7,6 +7,7 @@ cached, and all of them concatenated into one block-diagonal operator.
 """
 import sys
 
+import helpers as H
 import numpy as np
 import pytest
 
@@ -25,19 +26,7 @@ def say(*a):
 
 
 def oracle_sector_matrices(n, h, expr, sector_list):
-    O.lib().ora_group_set_norm_eps.argtypes = [O.C.c_void_p, O.C.c_double]
-    diag, off = expr.split()
-    out = []
-    for k, si in sector_list:
-        perms, num, den, re, im = O.group_closure(n, [models.chain_translation(n)], [k])
-        g = O.Group(n, perms, re, im, si or 0)
-        O.lib().ora_group_set_norm_eps(g.h, LS.DEFAULT_NORM_TOLERANCE)
-        lo = (1 << h) - 1
-        hi = lo << (n - h - 1) if si else lo << (n - h)
-        reps = g.enumerate(lo, hi)
-        M = O.matrix_dense(g, n, si or 0, reps, O.Terms(diag), O.Terms(off))
-        out.append((reps, M))
-    return out
+    return H.chain_sector_matrices(n, h, expr, sector_list, LS.DEFAULT_NORM_TOLERANCE)
 
 
 def test_sector_product_matches_oracle(engine):
