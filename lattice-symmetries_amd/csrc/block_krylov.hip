@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "operator_internal.hpp"
@@ -46,10 +47,11 @@ double seconds_since(Clock::time_point t) { return std::chrono::duration<double>
 constexpr int kChunkRows = 2048;
 
 // partial[(c * nblk + i) * B*B + p*B + q] = sum over rows r of chunk c of
-// A_i[r][p] * W[r][q], with A_i = A + i * a_stride.
-template <int B>
-__global__ __launch_bounds__(kThreads) void seg_gram_kernel(double const *__restrict__ A, int64_t a_stride, int nblk,
-                                                            double const *__restrict__ W,
+// A_i[r][p] * W[r][q], with A_i = A + i * a_stride.  Blocks are stored as TA /
+// TW (float or double); products and sums are FP64.
+template <int B, typename TA, typename TW>
+__global__ __launch_bounds__(kThreads) void seg_gram_kernel(TA const *__restrict__ A, int64_t a_stride, int nblk,
+                                                            TW const *__restrict__ W,
                                                             int64_t const *__restrict__ chunk_rows,
                                                             double *__restrict__ partial) {
   constexpr int V = B * B;
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(kThreads) void seg_gram_kernel(double const *__rest
   int64_t const r0 = chunk_rows[2 * c], r1 = chunk_rows[2 * c + 1];
   int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = 0; i < nblk; ++i) {
-    double const *__restrict__ Ai = A + static_cast<int64_t>(i) * a_stride;
+    TA const *__restrict__ Ai = A + static_cast<int64_t>(i) * a_stride;
     double acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0;
@@ -66,8 +68,8 @@ __global__ __launch_bounds__(kThreads) void seg_gram_kernel(double const *__rest
       double a[B], w[B];
 #pragma unroll
       for (int p = 0; p < B; ++p) {
-        a[p] = Ai[r * B + p];
-        w[p] = W[r * B + p];
+        a[p] = static_cast<double>(Ai[r * B + p]);
+        w[p] = static_cast<double>(W[r * B + p]);
       }
 #pragma unroll
       for (int p = 0; p < B; ++p)
@@ -112,10 +114,10 @@ __global__ __launch_bounds__(64) void seg_reduce_kernel(double const *__restrict
 }
 
 // out[r][q] = beta * out[r][q] + sum_{i < nblk} sum_p A_i[r][p] C_s[i][p][q]
-// for the segment s of row r.  Each thread owns whole rows, so out may alias
-// one of the A_i.
-template <int B>
-__global__ __launch_bounds__(kThreads) void seg_combine_kernel(double *out, double beta, double const *A,
+// for the segment s of row r, summed in FP64 and stored as TO.  Each thread
+// owns whole rows, so out may alias one of the A_i (when TO is TA).
+template <int B, typename TO, typename TA>
+__global__ __launch_bounds__(kThreads) void seg_combine_kernel(TO *out, double beta, TA const *A,
                                                                int64_t a_stride, int nblk,
                                                                double const *__restrict__ coef,
                                                                int64_t const *__restrict__ chunk_rows,
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(kThreads) void seg_combine_kernel(double *out, doub
     for (int i = 0; i < nblk; ++i) {
       double a[B];
 #pragma unroll
-      for (int p = 0; p < B; ++p) a[p] = A[static_cast<int64_t>(i) * a_stride + r * B + p];
+      for (int p = 0; p < B; ++p) a[p] = static_cast<double>(A[static_cast<int64_t>(i) * a_stride + r * B + p]);
 #pragma unroll
       for (int p = 0; p < B; ++p)
 #pragma unroll
@@ -143,17 +145,11 @@ __global__ __launch_bounds__(kThreads) void seg_combine_kernel(double *out, doub
     }
     if (beta != 0.0) {
 #pragma unroll
-      for (int q = 0; q < B; ++q) acc[q] = fma(beta, out[r * B + q], acc[q]);
+      for (int q = 0; q < B; ++q) acc[q] = fma(beta, static_cast<double>(out[r * B + q]), acc[q]);
     }
 #pragma unroll
-    for (int q = 0; q < B; ++q) out[r * B + q] = acc[q];
+    for (int q = 0; q < B; ++q) out[r * B + q] = static_cast<TO>(acc[q]);
   }
-}
-
-__global__ __launch_bounds__(kThreads) void to_f32_rows_kernel(double const *__restrict__ x, float *__restrict__ y,
-                                                               int64_t n) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i < n) y[i] = static_cast<float>(x[i]);
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -290,13 +286,14 @@ struct BlockAlgebra {
            d_coef.allocate(static_cast<size_t>(seg.nseg) * nblk_max * V);
   }
   // out_host[(s * nblk + i) * V + p * B + q] = A_i^T W over segment s
-  bool gram(double const *A, int64_t a_stride, int nblk, double const *W, std::vector<double> &out_host) {
+  template <typename TA, typename TW>
+  bool gram(TA const *A, int64_t a_stride, int nblk, TW const *W, std::vector<double> &out_host) {
     if (nblk > max_nblk) return fail("internal: too many blocks in a Gram product");
     int const V = B * B;
     if (seg.chunks > 0) {
       bool ok = dispatch_block(B, [&]<int BB>() -> bool {
-        seg_gram_kernel<BB><<<seg.chunks, kThreads, 0, stream()>>>(A, a_stride, nblk, W, seg.d_chunk_rows.ptr,
-                                                                   partial.ptr);
+        seg_gram_kernel<BB, TA, TW><<<seg.chunks, kThreads, 0, stream()>>>(A, a_stride, nblk, W,
+                                                                           seg.d_chunk_rows.ptr, partial.ptr);
         return true;
       });
       if (!ok) return false;
@@ -312,8 +309,8 @@ struct BlockAlgebra {
     return true;
   }
   // out = beta * out + A * C (per segment), coefficients [s][i][p][q] on the host
-  bool combine(double *out, double beta, double const *A, int64_t a_stride, int nblk,
-               std::vector<double> const &coef_host) {
+  template <typename TO, typename TA>
+  bool combine(TO *out, double beta, TA const *A, int64_t a_stride, int nblk, std::vector<double> const &coef_host) {
     if (nblk > max_nblk) return fail("internal: too many blocks in a block update");
     int const V = B * B;
     if (seg.chunks == 0) return true;
@@ -322,8 +319,8 @@ struct BlockAlgebra {
                              stream()));
     size_t const smem = sizeof(double) * static_cast<size_t>(nblk) * V;
     bool ok = dispatch_block(B, [&]<int BB>() -> bool {
-      seg_combine_kernel<BB><<<seg.chunks, kThreads, smem, stream()>>>(out, beta, A, a_stride, nblk, d_coef.ptr,
-                                                                       seg.d_chunk_rows.ptr, seg.d_chunk_seg.ptr);
+      seg_combine_kernel<BB, TO, TA><<<seg.chunks, kThreads, smem, stream()>>>(
+          out, beta, A, a_stride, nblk, d_coef.ptr, seg.d_chunk_rows.ptr, seg.d_chunk_seg.ptr);
       return true;
     });
     if (!ok) return false;
@@ -342,13 +339,6 @@ __global__ __launch_bounds__(kThreads) void column_kernel(double const *__restri
 }
 
 } // namespace
-
-bool block_to_f32(double const *x, float *y, int64_t count) {
-  if (count <= 0) return true;
-  to_f32_rows_kernel<<<grid_for(count, kThreads), kThreads, 0, stream()>>>(x, y, count);
-  LSAMD_HIP(hipGetLastError());
-  return true;
-}
 
 bool block_column(double const *X, int b, int col, double *out, int64_t rows) {
   if (rows <= 0) return true;
@@ -394,21 +384,27 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
   if (!alg.init(seg_rows, listed, B, m)) return false;
   int64_t const rows = alg.seg.rows;
   int64_t const blk = rows * B; // doubles per block
+  // FP64 buffers; while the product is FP32 the blocks are stored as FP32 in
+  // the same memory (block j at float offset j * blk).
   DeviceArray<double> Vb, W, Y;
   if (!Vb.allocate(static_cast<size_t>(blk) * m) || !W.allocate(static_cast<size_t>(blk)) ||
       !Y.allocate(static_cast<size_t>(blk) * o.keep))
     return false;
-  if (ns < nseg_all) {
-    // rows outside the solved segments are only ever read by the product,
-    // which keeps them inside their own (block-diagonal) segment
+  bool const partial_rows = ns < nseg_all;
+  // rows outside the solved segments are only ever read by the product,
+  // which keeps them inside their own (block-diagonal) segment
+  auto clear_unsolved = [&](bool with_y) -> bool {
+    if (!partial_rows) return true;
     LSAMD_HIP(hipMemsetAsync(Vb.ptr, 0, sizeof(double) * blk * m, stream()));
-    LSAMD_HIP(hipMemsetAsync(Y.ptr, 0, sizeof(double) * blk * o.keep, stream()));
-  }
-  auto Vj = [&](int j) { return Vb.ptr + static_cast<int64_t>(j) * blk; };
+    LSAMD_HIP(hipMemsetAsync(W.ptr, 0, sizeof(double) * blk, stream()));
+    if (with_y) LSAMD_HIP(hipMemsetAsync(Y.ptr, 0, sizeof(double) * blk * o.keep, stream()));
+    return true;
+  };
+  if (!clear_unsolved(true)) return false;
 
   double prod_s = 0.0;
   int64_t prods = 0;
-  auto apply = [&](bool fp32, double const *x, double *y) -> bool {
+  auto apply = [&](bool fp32, void const *x, void *y) -> bool {
     auto t0 = Clock::now();
     if (!product(fp32, x, y)) return false;
     LSAMD_HIP(hipStreamSynchronize(stream()));
@@ -418,11 +414,11 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
   };
   std::vector<double> G, Rf(static_cast<size_t>(ns) * V), coef;
   // CholQR2 of block X into out (may be X); Rf receives the factor R per segment.
-  auto cholqr = [&](double *X, double *out) -> bool {
+  auto cholqr = [&]<typename T>(T *X, T *out) -> bool {
     std::vector<double> Racc(static_cast<size_t>(ns) * V, 0.0);
     for (int s = 0; s < ns; ++s)
       for (int i = 0; i < B; ++i) Racc[static_cast<size_t>(s) * V + i * B + i] = 1.0;
-    double *src = X;
+    T *src = X;
     for (int pass = 0; pass < 2; ++pass) {
       if (!alg.gram(src, blk, 1, src, G)) return false;
       coef.assign(static_cast<size_t>(ns) * V, 0.0);
@@ -430,7 +426,7 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
         std::vector<double> Gs(G.begin() + static_cast<int64_t>(s) * V, G.begin() + static_cast<int64_t>(s + 1) * V);
         std::vector<double> R, Rinv;
         if (!cholesky_upper(Gs, B, R, Rinv))
-          return fail("block Lanczos breakdown in segment " + std::to_string(s) +
+          return fail("block Lanczos breakdown in segment " + std::to_string(listed[s]) +
                       " (the Krylov space is exhausted; lower max_blocks or the block size)");
         std::copy(Rinv.begin(), Rinv.end(), coef.begin() + static_cast<int64_t>(s) * V);
         std::vector<double> Ra(Racc.begin() + static_cast<int64_t>(s) * V,
@@ -445,126 +441,152 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
     return true;
   };
 
-  // start block: seeded N(0, 1), orthonormalised per segment
-  normal_kernel<double><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, Vj(0), blk);
-  LSAMD_HIP(hipGetLastError());
-  if (!cholqr(Vj(0), Vj(0))) return false;
-
   int const dimT = m * B;
-  std::vector<std::vector<double>> T(ns, std::vector<double>(static_cast<size_t>(dimT) * dimT, 0.0));
+  std::vector<std::vector<double>> Tp(ns, std::vector<double>(static_cast<size_t>(dimT) * dimT, 0.0));
   std::vector<std::vector<double>> theta(ns), Z(ns);
   std::vector<double> worst(ns, 0.0);
-  bool fp32 = o.fp32;
   int kept = 0;
-  int cycles_run = 0, fp64_cycles = 0, blocks_done = 0;
+  int cycle = 0, cycles_run = 0, fp64_cycles = 0, blocks_done = 0;
   double prev_worst = INFINITY;
   bool done = false;
   std::vector<double> C, Csum;
-  for (int cycle = 0; cycle < o.max_cycles; ++cycle) {
-    if (!fp32) ++fp64_cycles;
-    for (int j = kept; j < m; ++j) {
-      if (!apply(fp32, Vj(j), W.ptr)) return false;
-      ++blocks_done;
-      int const nb = j + 1;
-      Csum.assign(static_cast<size_t>(ns) * nb * V, 0.0);
-      for (int pass = 0; pass < 2; ++pass) {
-        if (!alg.gram(Vb.ptr, blk, nb, W.ptr, C)) return false;
-        for (size_t t = 0; t < C.size(); ++t) {
-          Csum[t] += C[t];
-          C[t] = -C[t];
-        }
-        if (!alg.combine(W.ptr, 1.0, Vb.ptr, blk, nb, C)) return false;
-      }
-      // column j of the projected matrix (and its mirror) from the coefficients
-      for (int s = 0; s < ns; ++s) {
-        std::vector<double> &Ts = T[s];
-        for (int i = 0; i < nb; ++i) {
-          double const *Ci = Csum.data() + (static_cast<size_t>(s) * nb + i) * V;
-          for (int p = 0; p < B; ++p)
-            for (int q = 0; q < B; ++q) {
-              size_t const r = static_cast<size_t>(i * B + p), c = static_cast<size_t>(j * B + q);
-              if (i == j) {
-                Ts[r * dimT + c] = 0.5 * (Ci[p * B + q] + Ci[q * B + p]);
-              } else {
-                Ts[r * dimT + c] = Ci[p * B + q];
-                Ts[c * dimT + r] = Ci[p * B + q];
-              }
-            }
-        }
-      }
-      if (j + 1 < m) {
-        if (!cholqr(W.ptr, Vj(j + 1))) return false;
-      } else {
-        if (!cholqr(W.ptr, W.ptr)) return false; // residual block; Rf holds its factor
-      }
-    }
-    // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
-    std::vector<int> eig_ok(ns, 1);
-    parallel_for(ns, [&](int s) {
-      eig_ok[s] = symmetric_eigh(T[s], dimT, theta[s], Z[s]) ? 1 : 0;
-      if (!eig_ok[s]) return;
-      double w = 0.0;
-      double const *Rs = Rf.data() + static_cast<size_t>(s) * V;
-      for (int q = 0; q < want; ++q) {
-        double rs = 0.0;
-        for (int p = 0; p < B; ++p) {
-          double acc = 0.0;
-          for (int c = 0; c < B; ++c) acc += Rs[p * B + c] * Z[s][static_cast<size_t>((m - 1) * B + c) * dimT + q];
-          rs += acc * acc;
-        }
-        w = std::max(w, std::sqrt(rs) / std::max(1.0, std::fabs(theta[s][q])));
-      }
-      worst[s] = w;
-    });
+  // coefficients of Ritz block t: coef[s][i] = Z_s[i-block rows, t-block columns]
+  auto ritz_coef = [&](int t) {
+    coef.assign(static_cast<size_t>(ns) * m * V, 0.0);
     for (int s = 0; s < ns; ++s)
-      if (!eig_ok[s]) return fail("projected eigenproblem did not converge in segment " + std::to_string(s));
-    double wmax = 0.0;
-    for (int s = 0; s < ns; ++s) wmax = std::max(wmax, worst[s]);
-    ++cycles_run;
-    res.history.push_back(wmax);
-    if (!fp32 && wmax < o.tol) {
-      done = true;
-      break;
-    }
-    if (cycle + 1 == o.max_cycles) break; // keep V and Z for the final Ritz block
-    // FP32 gives way to FP64 at the switch tolerance or when FP32 stalls
-    bool const to_fp64 = fp32 && (wmax < o.switch_tol || wmax > 0.9 * prev_worst);
-    prev_worst = wmax;
-    // Restart: Y_t = sum_i V_i Z_s[i-block, t-block]
-    int const r = to_fp64 ? 1 : o.keep;
-    for (int t = 0; t < r; ++t) {
-      coef.assign(static_cast<size_t>(ns) * m * V, 0.0);
+      for (int i = 0; i < m; ++i)
+        for (int p = 0; p < B; ++p)
+          for (int q = 0; q < B; ++q)
+            coef[(static_cast<size_t>(s) * m + i) * V + p * B + q] =
+                Z[s][static_cast<size_t>(i * B + p) * dimT + t * B + q];
+  };
+
+  // Restart cycles with the blocks stored as T: float runs the FP32 product,
+  // double the FP64 product.  `kept` blocks (and their part of Tp) are in
+  // place on entry.  Returns 1 when FP32 gives way to FP64, 0 when converged
+  // or out of cycles, -1 on error; on 0 and 1 the lowest Ritz block is in Y
+  // as FP64.
+  auto run_cycles = [&]<typename T>(T *Vbase, T *Wt, T *Yt) -> int {
+    constexpr bool fp32 = std::is_same_v<T, float>;
+    auto Vj = [&](int j) { return Vbase + static_cast<int64_t>(j) * blk; };
+    for (; cycle < o.max_cycles; ++cycle) {
+      if (!fp32) ++fp64_cycles;
+      for (int j = kept; j < m; ++j) {
+        if (!apply(fp32, Vj(j), Wt)) return -1;
+        ++blocks_done;
+        int const nb = j + 1;
+        Csum.assign(static_cast<size_t>(ns) * nb * V, 0.0);
+        for (int pass = 0; pass < 2; ++pass) {
+          if (!alg.gram(Vbase, blk, nb, Wt, C)) return -1;
+          for (size_t t = 0; t < C.size(); ++t) {
+            Csum[t] += C[t];
+            C[t] = -C[t];
+          }
+          if (!alg.combine(Wt, 1.0, Vbase, blk, nb, C)) return -1;
+        }
+        // column j of the projected matrix (and its mirror) from the coefficients
+        for (int s = 0; s < ns; ++s) {
+          std::vector<double> &Ts = Tp[s];
+          for (int i = 0; i < nb; ++i) {
+            double const *Ci = Csum.data() + (static_cast<size_t>(s) * nb + i) * V;
+            for (int p = 0; p < B; ++p)
+              for (int q = 0; q < B; ++q) {
+                size_t const r = static_cast<size_t>(i * B + p), c = static_cast<size_t>(j * B + q);
+                if (i == j) {
+                  Ts[r * dimT + c] = 0.5 * (Ci[p * B + q] + Ci[q * B + p]);
+                } else {
+                  Ts[r * dimT + c] = Ci[p * B + q];
+                  Ts[c * dimT + r] = Ci[p * B + q];
+                }
+              }
+          }
+        }
+        if (j + 1 < m) {
+          if (!cholqr(Wt, Vj(j + 1))) return -1;
+        } else {
+          if (!cholqr(Wt, Wt)) return -1; // residual block; Rf holds its factor
+        }
+      }
+      // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
+      std::vector<int> eig_ok(ns, 1);
+      parallel_for(ns, [&](int s) {
+        eig_ok[s] = symmetric_eigh(Tp[s], dimT, theta[s], Z[s]) ? 1 : 0;
+        if (!eig_ok[s]) return;
+        double w = 0.0;
+        double const *Rs = Rf.data() + static_cast<size_t>(s) * V;
+        for (int q = 0; q < want; ++q) {
+          double rs = 0.0;
+          for (int p = 0; p < B; ++p) {
+            double acc = 0.0;
+            for (int c = 0; c < B; ++c) acc += Rs[p * B + c] * Z[s][static_cast<size_t>((m - 1) * B + c) * dimT + q];
+            rs += acc * acc;
+          }
+          w = std::max(w, std::sqrt(rs) / std::max(1.0, std::fabs(theta[s][q])));
+        }
+        worst[s] = w;
+      });
       for (int s = 0; s < ns; ++s)
-        for (int i = 0; i < m; ++i)
-          for (int p = 0; p < B; ++p)
-            for (int q = 0; q < B; ++q)
-              coef[(static_cast<size_t>(s) * m + i) * V + p * B + q] =
-                  Z[s][static_cast<size_t>(i * B + p) * dimT + t * B + q];
-      if (!alg.combine(Y.ptr + static_cast<int64_t>(t) * blk, 0.0, Vb.ptr, blk, m, coef)) return false;
-    }
-    LSAMD_HIP(hipMemcpyAsync(Vb.ptr, Y.ptr, sizeof(double) * blk * r, hipMemcpyDeviceToDevice, stream()));
-    for (int s = 0; s < ns; ++s) std::fill(T[s].begin(), T[s].end(), 0.0);
-    if (to_fp64) {
-      // explicit restart from the lowest Ritz block with the FP64 product
-      fp32 = false;
-      kept = 0;
-      if (!cholqr(Vj(0), Vj(0))) return false;
-    } else {
-      LSAMD_HIP(hipMemcpyAsync(Vj(r), W.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
-      for (int s = 0; s < ns; ++s)
-        for (int q = 0; q < r * B; ++q) T[s][static_cast<size_t>(q) * dimT + q] = theta[s][q];
+        if (!eig_ok[s])
+          return fail("projected eigenproblem did not converge in segment " + std::to_string(listed[s])) ? 0 : -1;
+      double wmax = 0.0;
+      for (int s = 0; s < ns; ++s) wmax = std::max(wmax, worst[s]);
+      ++cycles_run;
+      res.history.push_back(wmax);
+      done = !fp32 && wmax < o.tol;
+      bool const last = cycle + 1 == o.max_cycles;
+      // FP32 gives way to FP64 at the switch tolerance or when FP32 stalls
+      bool const to_fp64 = fp32 && !last && (wmax < o.switch_tol || wmax > 0.9 * prev_worst);
+      prev_worst = wmax;
+      if (done || last || to_fp64) {
+        ritz_coef(0);
+        if (!alg.combine(Y.ptr, 0.0, Vbase, blk, m, coef)) return -1;
+        ++cycle;
+        return to_fp64 ? 1 : 0;
+      }
+      // Restart: Y_t = sum_i V_i Z_s[i-block, t-block], then the residual block
+      int const r = o.keep;
+      for (int t = 0; t < r; ++t) {
+        ritz_coef(t);
+        if (!alg.combine(Yt + static_cast<int64_t>(t) * blk, 0.0, Vbase, blk, m, coef)) return -1;
+      }
+      LSAMD_HIP(hipMemcpyAsync(Vbase, Yt, sizeof(T) * blk * r, hipMemcpyDeviceToDevice, stream()));
+      LSAMD_HIP(hipMemcpyAsync(Vj(r), Wt, sizeof(T) * blk, hipMemcpyDeviceToDevice, stream()));
+      for (int s = 0; s < ns; ++s) {
+        std::fill(Tp[s].begin(), Tp[s].end(), 0.0);
+        for (int q = 0; q < r * B; ++q) Tp[s][static_cast<size_t>(q) * dimT + q] = theta[s][q];
+      }
       kept = r;
+      LSAMD_HIP(hipStreamSynchronize(stream()));
     }
-    LSAMD_HIP(hipStreamSynchronize(stream()));
+    return 0;
+  };
+
+  int status = 0;
+  if (o.fp32) {
+    float *V32 = reinterpret_cast<float *>(Vb.ptr);
+    normal_kernel<float><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, V32, blk);
+    LSAMD_HIP(hipGetLastError());
+    if (!cholqr(V32, V32)) return false;
+    status = run_cycles(V32, reinterpret_cast<float *>(W.ptr), reinterpret_cast<float *>(Y.ptr));
+    if (status < 0) return false;
+    if (status == 1) {
+      // explicit restart from the lowest Ritz block (in Y) with the FP64
+      // product; Y's unsolved rows are still zero
+      LSAMD_HIP(hipStreamSynchronize(stream()));
+      if (!clear_unsolved(false)) return false;
+      LSAMD_HIP(hipMemcpyAsync(Vb.ptr, Y.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
+      for (int s = 0; s < ns; ++s) std::fill(Tp[s].begin(), Tp[s].end(), 0.0);
+      kept = 0;
+      if (!cholqr(Vb.ptr, Vb.ptr)) return false;
+    }
+  } else {
+    normal_kernel<double><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, Vb.ptr, blk);
+    LSAMD_HIP(hipGetLastError());
+    if (!cholqr(Vb.ptr, Vb.ptr)) return false;
+    status = 1;
   }
-  // Final: the lowest Ritz block, its FP64 Rayleigh quotients and residuals.
-  coef.assign(static_cast<size_t>(ns) * m * V, 0.0);
-  for (int s = 0; s < ns; ++s)
-    for (int i = 0; i < m; ++i)
-      for (int p = 0; p < B; ++p)
-        for (int q = 0; q < B; ++q)
-          coef[(static_cast<size_t>(s) * m + i) * V + p * B + q] = Z[s][static_cast<size_t>(i * B + p) * dimT + q];
-  if (!alg.combine(Y.ptr, 0.0, Vb.ptr, blk, m, coef)) return false;
+  if (status == 1 && run_cycles(Vb.ptr, W.ptr, Y.ptr) < 0) return false;
+  // Final: the lowest Ritz block (in Y), its FP64 Rayleigh quotients and residuals.
   if (!apply(false, Y.ptr, W.ptr)) return false;
   std::vector<double> Th, Gr;
   if (!alg.gram(Y.ptr, blk, 1, W.ptr, Th)) return false; // Y^T A Y

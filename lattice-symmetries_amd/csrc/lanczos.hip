@@ -455,18 +455,12 @@ extern "C" int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int 
 
 namespace {
 // Block Lanczos on one operator through the segmented solver (one segment).
-bool operator_block_solve(OperatorCore &core, int mode, BlockKrylovOptions const &o, BlockKrylovResult &r,
+bool operator_block_solve(OperatorCore &core, BlockKrylovOptions const &o, BlockKrylovResult &r,
                           double *d_ritz_block) {
   int64_t const n = core.number_states;
-  int64_t const count = n * o.block;
-  DeviceArray<float> X32;
-  if (mode == 1 && !X32.allocate(static_cast<size_t>(std::max<int64_t>(count, 1)))) return false;
-  BlockProduct prod = [&](bool fp32, double const *x, double *y) -> bool {
-    if (fp32) {
-      if (!block_to_f32(x, X32.ptr, count)) return false;
-      return operator_csr_f32_block_out64(core, o.block, X32.ptr, y);
-    }
-    return operator_csr_f64_block(core, o.block, x, y);
+  BlockProduct prod = [&](bool fp32, void const *x, void *y) -> bool {
+    if (fp32) return operator_csr_f32_block(core, o.block, static_cast<float const *>(x), static_cast<float *>(y));
+    return operator_csr_f64_block(core, o.block, static_cast<double const *>(x), static_cast<double *>(y));
   };
   std::vector<int64_t> const seg{0, n};
   return block_krylov_solve(seg, {}, 1, prod, o, r, d_ritz_block);
@@ -508,7 +502,7 @@ extern "C" int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_
   DeviceArray<double> Yb;
   if (d_ritz_out != nullptr && !Yb.allocate(static_cast<size_t>(n) * block_size)) return -1;
   BlockKrylovResult r;
-  if (!operator_block_solve(op->core, mode, o, r, d_ritz_out != nullptr ? Yb.ptr : nullptr)) return -1;
+  if (!operator_block_solve(op->core, o, r, d_ritz_out != nullptr ? Yb.ptr : nullptr)) return -1;
   std::copy(r.evals.begin(), r.evals.end(), evals);
   if (residuals != nullptr) std::copy(r.residuals.begin(), r.residuals.end(), residuals);
   if (d_ritz_out != nullptr) {
@@ -542,7 +536,7 @@ extern "C" int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_siz
   DeviceArray<double> Yb, u;
   if (!Yb.allocate(static_cast<size_t>(n) * block_size) || !u.allocate(static_cast<size_t>(n))) return -1;
   BlockKrylovResult r;
-  if (!operator_block_solve(op->core, mode, o, r, Yb.ptr)) return -1;
+  if (!operator_block_solve(op->core, o, r, Yb.ptr)) return -1;
   if (!block_column(Yb.ptr, block_size, 0, u.ptr, n)) return -1;
   std::vector<double> evals = r.evals;
   double mv = r.product_seconds;

@@ -952,32 +952,6 @@ bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_
   return ok;
 }
 
-bool operator_csr_f32_block_out64(OperatorCore &op, int k, float const *d_x, double *d_y) {
-  Csr const &c = op.csr;
-  if (c.is_complex) return fail("the CSR holds complex values: use a sector batch for its products");
-  if (!c.has32) return fail("FP32 CSR has not been built");
-  int64_t const n = c.rows;
-  if (n == 0) return true;
-  timer_begin(kTimerCsrF32);
-  bool ok = dispatch_k(k, [&]<int K>() -> bool {
-    return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
-      int const grid = product_grid(op, n, LPR, op.flags_f32);
-      return dispatch_flags(op.flags_f32, [&]<bool NT, bool DIE>() -> bool {
-        if (c.packed)
-          csr_block_kernel<K, LPR, float, double, NT, DIE, true><<<grid, 256, 0, stream()>>>(
-              n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table32.ptr, c.table_size, c.diag32.ptr, d_x, d_y);
-        else
-          csr_block_kernel<K, LPR, float, double, NT, DIE, false><<<grid, 256, 0, stream()>>>(
-              n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, nullptr, 0, c.diag32.ptr, d_x, d_y);
-        LSAMD_HIP(hipGetLastError());
-        return true;
-      });
-    });
-  });
-  timer_end(kTimerCsrF32);
-  return ok;
-}
-
 bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *d_y) {
   Csr const &c = op.csr;
   if (c.is_complex) return fail("the CSR holds complex values: use a sector batch for its products");

@@ -93,9 +93,6 @@ bool apply_off_diag_device(Terms const &t, int64_t count, uint64_t const *d_alph
 
 bool exclusive_scan(uint32_t const *d_counts, int64_t n, uint64_t *d_offsets, uint64_t &total);
 
-// FP32 block of k right-hand sides in, FP64 out (row-major [n][k]).
-bool operator_csr_f32_block_out64(OperatorCore &op, int k, float const *d_x, double *d_y);
-
 // Thick-restart block Lanczos on FP64 row-major [rows][block] blocks over
 // independent segments of rows (block_krylov.hip).
 struct BlockKrylovOptions {
@@ -121,8 +118,9 @@ struct BlockKrylovResult {
   double first_rayleigh = 0.0, first_residual = 0.0; // lowest Ritz vector of segment 0
 };
 
-// y = A x on a block; fp32 selects the FP32 product (FP64 x and y).
-using BlockProduct = std::function<bool(bool fp32, double const *x, double *y)>;
+// y = A x on a block; with fp32 the FP32 product on float blocks, otherwise
+// the FP64 product on double blocks.
+using BlockProduct = std::function<bool(bool fp32, void const *x, void *y)>;
 
 // Each eigenvalue appears `multiplicity` times in the segments' spectra (2 for
 // the real embedding of complex sectors); d_ritz_block (may be null) receives
@@ -133,7 +131,6 @@ bool check_block_krylov_options(BlockKrylovOptions const &o, int multiplicity);
 bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const &active, int multiplicity,
                         BlockProduct const &product, BlockKrylovOptions const &o, BlockKrylovResult &res,
                         double *d_ritz_block);
-bool block_to_f32(double const *x, float *y, int64_t count);
 
 // Dense symmetric eigen-decomposition (dense_eigh.cpp): A is n x n row-major;
 // w receives the eigenvalues ascending and Z the eigenvectors as columns.

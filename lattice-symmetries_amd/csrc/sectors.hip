@@ -37,15 +37,25 @@ struct ls_amd_sector_batch {
   int lanes = 4;
   int blocks = 2048;
   int xcd_aware = 1;
-  // graph of the last product launch
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  void const *g_x = nullptr;
-  void *g_y = nullptr;
-  int g_k = 0, g_form = -1, g_lanes = 0, g_blocks = 0, g_xcd = -1;
+  // instantiated graphs of recent product launches, one kernel node each
+  struct Graph {
+    void const *x = nullptr;
+    void *y = nullptr;
+    int k = 0, form = -1, lanes = 0, blocks = 0, xcd = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+  };
+  std::vector<Graph> graphs;
+  uint64_t graph_clock = 0;
+  static void release(Graph &g) {
+    if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph != nullptr) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+  }
   ~ls_amd_sector_batch() {
-    if (exec != nullptr) (void)hipGraphExecDestroy(exec);
-    if (graph != nullptr) (void)hipGraphDestroy(graph);
+    for (Graph &g : graphs) release(g);
   }
 };
 
@@ -65,7 +75,7 @@ template <> struct Pair<double> { using type = double2; };
 
 // y[i] = d_i x[i] + sum_k M_{i, cols[k]} x[cols[k]], for K complex columns.
 // vals[2k], vals[2k + 1] are (re, im) of M_{i, cols[k]}.  x is T, y is TO
-// (FP32 products may write FP64 output directly).
+// (both float or both double as launched).
 template <int K, int LPR, typename T, typename TO>
 __global__ __launch_bounds__(kThreads) void sector_spmv_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                                uint32_t const *__restrict__ cols,
@@ -159,7 +169,7 @@ template <typename F> bool dispatch_columns(int k, F &&f) {
   }
 }
 
-// form: 0 FP64 in / out, 1 FP32 in / out, 2 FP32 in / FP64 out
+// form: 0 FP64 in / out, 1 FP32 in / out
 bool launch_product(ls_amd_sector_batch const &b, int form, int k, void const *x, void *y, int lanes, int blocks,
                     int xcd) {
   Csr const &c = b.csr;
@@ -171,14 +181,10 @@ bool launch_product(ls_amd_sector_batch const &b, int form, int k, void const *x
         sector_spmv_kernel<K, LPR, double, double><<<blocks, kThreads, 0, stream()>>>(
             n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, static_cast<double const *>(x),
             static_cast<double *>(y), xcd);
-      } else if (form == 1) {
+      } else {
         sector_spmv_kernel<K, LPR, float, float><<<blocks, kThreads, 0, stream()>>>(
             n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, static_cast<float const *>(x),
             static_cast<float *>(y), xcd);
-      } else {
-        sector_spmv_kernel<K, LPR, float, double><<<blocks, kThreads, 0, stream()>>>(
-            n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, static_cast<float const *>(x),
-            static_cast<double *>(y), xcd);
       }
       return true;
     });
@@ -193,40 +199,62 @@ bool check_schedule(int lanes, int blocks, int xcd) {
   return true;
 }
 
-// The product through the cached graph: captured on first use of a given
-// (form, k, x, y, schedule) and replayed afterwards.  The graph holds the one
-// kernel node and no memory operations.
+// The product through a cached graph: captured on first use of a given
+// (form, k, x, y, schedule) and replayed afterwards.  Each graph holds the one
+// kernel node and no memory operations; the least recently used of
+// kGraphCache graphs is dropped to make room (a Krylov solve cycles through
+// one input block per Krylov block).
+constexpr size_t kGraphCache = 64;
+
 bool graph_product(ls_amd_sector_batch &b, int form, int k, void const *x, void *y) {
   if (!check_schedule(b.lanes, b.blocks, b.xcd_aware)) return false;
   if (b.csr.rows == 0) return true;
-  bool const same = b.exec != nullptr && b.g_x == x && b.g_y == y && b.g_k == k && b.g_form == form &&
-                    b.g_lanes == b.lanes && b.g_blocks == b.blocks && b.g_xcd == b.xcd_aware;
-  if (!same) {
-    if (b.exec != nullptr) (void)hipGraphExecDestroy(b.exec);
-    if (b.graph != nullptr) (void)hipGraphDestroy(b.graph);
-    b.exec = nullptr;
-    b.graph = nullptr;
+  using Graph = ls_amd_sector_batch::Graph;
+  Graph *hit = nullptr;
+  for (Graph &g : b.graphs)
+    if (g.x == x && g.y == y && g.k == k && g.form == form && g.lanes == b.lanes && g.blocks == b.blocks &&
+        g.xcd == b.xcd_aware) {
+      hit = &g;
+      break;
+    }
+  if (hit == nullptr) {
     LSAMD_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
     bool const ok = launch_product(b, form, k, x, y, b.lanes, b.blocks, b.xcd_aware);
-    hipGraph_t g = nullptr;
-    hipError_t const e = hipStreamEndCapture(stream(), &g);
+    hipGraph_t gr = nullptr;
+    hipError_t const e = hipStreamEndCapture(stream(), &gr);
     if (!ok) {
-      if (g != nullptr) (void)hipGraphDestroy(g);
+      if (gr != nullptr) (void)hipGraphDestroy(gr);
       return false;
     }
     LSAMD_HIP(e);
-    b.graph = g;
-    LSAMD_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
-    b.g_x = x;
-    b.g_y = y;
-    b.g_k = k;
-    b.g_form = form;
-    b.g_lanes = b.lanes;
-    b.g_blocks = b.blocks;
-    b.g_xcd = b.xcd_aware;
+    hipGraphExec_t ex = nullptr;
+    hipError_t const ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(gr);
+      LSAMD_HIP(ei);
+    }
+    if (b.graphs.size() >= kGraphCache) {
+      auto lru = std::min_element(b.graphs.begin(), b.graphs.end(),
+                                  [](Graph const &p, Graph const &q) { return p.last_use < q.last_use; });
+      ls_amd_sector_batch::release(*lru);
+      b.graphs.erase(lru);
+    }
+    Graph g;
+    g.x = x;
+    g.y = y;
+    g.k = k;
+    g.form = form;
+    g.lanes = b.lanes;
+    g.blocks = b.blocks;
+    g.xcd = b.xcd_aware;
+    g.graph = gr;
+    g.exec = ex;
+    b.graphs.push_back(g);
+    hit = &b.graphs.back();
   }
+  hit->last_use = ++b.graph_clock;
   timer_begin(kTimerSectorProduct);
-  LSAMD_HIP(hipGraphLaunch(b.exec, stream()));
+  LSAMD_HIP(hipGraphLaunch(hit->exec, stream()));
   timer_end(kTimerSectorProduct);
   return true;
 }
@@ -452,15 +480,9 @@ bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &o, Blo
     }
   }
   if (dense < ns) {
-    int64_t const count = 2 * bt.csr.rows * o.block;
-    DeviceArray<float> X32;
-    if (!X32.allocate(static_cast<size_t>(std::max<int64_t>(count, 1)))) return false;
     int const B = o.block;
-    BlockProduct prod = [&](bool fp32, double const *x, double *y) -> bool {
-      if (fp32) {
-        if (!block_to_f32(x, X32.ptr, count)) return false;
-        return graph_product(bt, 2, B, X32.ptr, y);
-      }
+    BlockProduct prod = [&](bool fp32, void const *x, void *y) -> bool {
+      if (fp32) return graph_product(bt, 1, B, x, y);
       if (!launch_product(bt, 0, B, x, y, bt.lanes, bt.blocks, bt.xcd_aware)) return false;
       LSAMD_HIP(hipGetLastError());
       return true;
