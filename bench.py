@@ -129,7 +129,7 @@ def sectors_bench(args):
     traffic, traffic_src = profiled_traffic(f"chain{n}_sectors", sector_kernel)
     value = 8.0 * nnz * args.steps / t_steps
     del dx, dy
-    r = batch.eigensolve(nev=4, block_size=8, max_blocks=args.sector_blocks, keep_blocks=args.sector_keep,
+    r = batch.eigensolve(nev=4, block_size=4, max_blocks=args.sector_blocks, keep_blocks=args.sector_keep,
                          max_cycles=args.sector_cycles, tol=1e-7, switch_tol=1e-5, seed=1)
     log(f"eigensolve: {r['cycles']} cycles ({r['fp64_cycles']} FP64), {r['products']} products, "
         f"{r['total_seconds']:.1f} s, max residual {r['max_residual']:.2e}")
@@ -165,7 +165,10 @@ def sectors_bench(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": sector_kernel, "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch},
         "setup_seconds": t_setup,
-        "eigensolve": {"cycles": r["cycles"], "fp64_cycles": r["fp64_cycles"], "products": r["products"],
+        "eigensolve": {"method": "complex thick-restart block Lanczos, FP32 then FP64",
+                       "block_complex_columns": 4, "max_blocks": args.sector_blocks,
+                       "keep_blocks": args.sector_keep,
+                       "cycles": r["cycles"], "fp64_cycles": r["fp64_cycles"], "products": r["products"],
                        "product_seconds": r["product_seconds"], "seconds": r["total_seconds"],
                        "max_residual": r["max_residual"], "converged": r["converged"],
                        "E0_k0": float(ev[s0][0]), "E0_kpi": float(ev[spi][0]), "E0_exact": mg,
@@ -234,9 +237,9 @@ def main():
                     default="square6")
     ap.add_argument("--copy-mib", type=int, default=2048)
     ap.add_argument("--chain", type=int, default=28)
-    ap.add_argument("--sector-blocks", type=int, default=12)
+    ap.add_argument("--sector-blocks", type=int, default=16)
     ap.add_argument("--sector-cycles", type=int, default=40)
-    ap.add_argument("--sector-keep", type=int, default=3)
+    ap.add_argument("--sector-keep", type=int, default=4)
     ap.add_argument("--reference-sectors", default="0+,14+,7-,13+")
     ap.add_argument("--reference-steps", type=int, default=400)
     args = ap.parse_args()

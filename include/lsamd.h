@@ -392,6 +392,12 @@ int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size,
    matrices.  w[n] ascending; Z (may be NULL) the eigenvectors as columns. */
 int ls_amd_dense_eigh(int n, double const *A, double *w, double *Z);
 
+/* Host utility: the same for a dense complex Hermitian n x n matrix A
+   (row-major, interleaved re, im), by cyclic Jacobi with complex rotations,
+   as the block solver uses for the projected matrices of complex sectors.
+   w[n] ascending; Z (may be NULL) the eigenvectors as columns, interleaved. */
+int ls_amd_dense_eigh_hermitian(int n, double const *A, double *w, double *Z);
+
 /* ---- Many symmetry sectors in one product --------------------------------
    No counterpart in the reference, which builds one basis per sector and
    diagonalises the sectors one after another (chapel/src/Diagonalize.chpl:
@@ -430,15 +436,18 @@ int ls_amd_sector_batch_matvec_f64(ls_amd_sector_batch *batch, int k,
                                    double const *d_x, double *d_y);
 
 /* Thick-restart block Lanczos (as ls_amd_block_eigensolve) on every sector
-   at once, one product launch per block step, on the real
-   embedding with block_size (1, 2, 4, 8) columns; nev eigenvalues per
-   sector (2 nev <= block_size).  max_blocks Krylov blocks per cycle, the
-   lowest keep_blocks Ritz blocks kept at a restart.  The product is FP32
-   (FP64 Krylov vectors and orthogonalisation) until the relative residual
-   falls below switch_tol or stalls, then FP64 until tol.  evals[count *
-   nev] receives each sector's lowest eigenvalues ascending; residuals (may
-   be NULL) their FP64 relative residual norms || M u - lambda u || /
-   max(1, |lambda|), computed with the FP64 product at the end. */
+   at once, one product launch per block step, in complex arithmetic with
+   blocks of block_size (1, 2, 4) complex columns; nev eigenvalues per
+   sector (nev <= block_size).  max_blocks Krylov blocks per cycle, the
+   lowest keep_blocks Ritz blocks kept at a restart.  The product is FP32,
+   on Krylov blocks stored as FP32 with FP64 orthogonalisation sums, until
+   the relative residual falls below switch_tol or stalls, then FP64 on
+   FP64 blocks until tol.  Sectors with fewer than 2 (max_blocks + 1)
+   block_size states (up to 512) are diagonalised densely instead.
+   evals[count * nev] receives each sector's lowest eigenvalues ascending
+   (NaN past a sector's dimension); residuals (may be NULL) their FP64
+   relative residual norms || M u - lambda u || / max(1, |lambda|), computed
+   with the FP64 product at the end. */
 int ls_amd_sector_batch_eigensolve(ls_amd_sector_batch *batch, int block_size,
                                    int nev, int max_blocks, int keep_blocks,
                                    int max_cycles, double tol,

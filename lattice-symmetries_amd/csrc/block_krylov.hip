@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -211,33 +212,86 @@ struct Segments {
   }
 };
 
-// Host-side dense helpers for the per-segment small matrices.
-// Cholesky G = L L^T of a B x B SPD matrix; returns false if a pivot is not
-// positive (rank deficiency).  R = L^T (upper), Rinv = R^{-1}.
-bool cholesky_upper(std::vector<double> const &G, int B, std::vector<double> &R, std::vector<double> &Rinv) {
-  std::vector<double> L(static_cast<size_t>(B) * B, 0.0);
+// Host-side dense helpers for the per-segment small matrices, over the field
+// F of the Krylov process: double, or std::complex<double> for blocks of
+// complex columns stored as real rows [re | im].
+template <typename F> struct Field;
+template <> struct Field<double> {
+  static constexpr int kWidth = 1; // reals per entry of a block row
+  static double conj(double x) { return x; }
+  static double re(double x) { return x; }
+};
+template <> struct Field<std::complex<double>> {
+  static constexpr int kWidth = 2;
+  static std::complex<double> conj(std::complex<double> x) { return std::conj(x); }
+  static double re(std::complex<double> x) { return x.real(); }
+};
+
+// `count` real Bw x Bw Gram matrices X^T Y of blocks with rows [re | im] ->
+// the field's Bf x Bf matrices X^H Y (Bw = Bf * width).
+template <typename F> void gram_to_field(std::vector<double> const &G, size_t count, int Bf, std::vector<F> &out) {
+  size_t const Vf = static_cast<size_t>(Bf) * Bf;
+  out.resize(count * Vf);
+  if constexpr (Field<F>::kWidth == 1) {
+    std::copy(G.begin(), G.begin() + static_cast<std::ptrdiff_t>(count * Vf), out.begin());
+  } else {
+    int const Bw = 2 * Bf;
+    size_t const Vw = static_cast<size_t>(Bw) * Bw;
+    for (size_t k = 0; k < count; ++k) {
+      double const *g = G.data() + k * Vw;
+      F *o = out.data() + k * Vf;
+      for (int a = 0; a < Bf; ++a)
+        for (int b = 0; b < Bf; ++b)
+          o[a * Bf + b] = {g[a * Bw + b] + g[(Bf + a) * Bw + Bf + b], g[a * Bw + Bf + b] - g[(Bf + a) * Bw + b]};
+    }
+  }
+}
+
+// The real Bw x Bw coefficients with which the block update applies the
+// field matrix M (row vector [re | im] times [[Re M, Im M], [-Im M, Re M]]).
+template <typename F> void field_to_coef(F const *M, int Bf, double *out) {
+  if constexpr (Field<F>::kWidth == 1) {
+    std::copy(M, M + static_cast<size_t>(Bf) * Bf, out);
+  } else {
+    int const Bw = 2 * Bf;
+    for (int a = 0; a < Bf; ++a)
+      for (int b = 0; b < Bf; ++b) {
+        double const r = M[a * Bf + b].real(), i = M[a * Bf + b].imag();
+        out[a * Bw + b] = r;
+        out[a * Bw + Bf + b] = i;
+        out[(Bf + a) * Bw + b] = -i;
+        out[(Bf + a) * Bw + Bf + b] = r;
+      }
+  }
+}
+
+// Cholesky G = R^H R of a B x B Hermitian positive definite matrix; returns
+// false if a pivot is not positive (rank deficiency).  R upper, Rinv = R^-1.
+template <typename F> bool cholesky_upper(std::vector<F> const &G, int B, std::vector<F> &R, std::vector<F> &Rinv) {
+  std::vector<F> L(static_cast<size_t>(B) * B, F(0.0));
   double scale = 0.0;
-  for (int i = 0; i < B; ++i) scale = std::max(scale, G[static_cast<size_t>(i) * B + i]);
+  for (int i = 0; i < B; ++i) scale = std::max(scale, Field<F>::re(G[static_cast<size_t>(i) * B + i]));
   for (int j = 0; j < B; ++j) {
-    double d = G[static_cast<size_t>(j) * B + j];
-    for (int k = 0; k < j; ++k) d -= L[static_cast<size_t>(j) * B + k] * L[static_cast<size_t>(j) * B + k];
+    double d = Field<F>::re(G[static_cast<size_t>(j) * B + j]);
+    for (int k = 0; k < j; ++k) d -= std::norm(L[static_cast<size_t>(j) * B + k]);
     if (!(d > 1e-28 * std::max(scale, 1e-300))) return false;
     double const ljj = std::sqrt(d);
     L[static_cast<size_t>(j) * B + j] = ljj;
     for (int i = j + 1; i < B; ++i) {
-      double v = G[static_cast<size_t>(i) * B + j];
-      for (int k = 0; k < j; ++k) v -= L[static_cast<size_t>(i) * B + k] * L[static_cast<size_t>(j) * B + k];
+      F v = G[static_cast<size_t>(i) * B + j];
+      for (int k = 0; k < j; ++k)
+        v -= L[static_cast<size_t>(i) * B + k] * Field<F>::conj(L[static_cast<size_t>(j) * B + k]);
       L[static_cast<size_t>(i) * B + j] = v / ljj;
     }
   }
-  R.assign(static_cast<size_t>(B) * B, 0.0);
+  R.assign(static_cast<size_t>(B) * B, F(0.0));
   for (int i = 0; i < B; ++i)
-    for (int j = 0; j < B; ++j) R[static_cast<size_t>(i) * B + j] = L[static_cast<size_t>(j) * B + i];
+    for (int j = 0; j < B; ++j) R[static_cast<size_t>(i) * B + j] = Field<F>::conj(L[static_cast<size_t>(j) * B + i]);
   // Rinv by back substitution, column by column
-  Rinv.assign(static_cast<size_t>(B) * B, 0.0);
+  Rinv.assign(static_cast<size_t>(B) * B, F(0.0));
   for (int col = 0; col < B; ++col) {
     for (int i = B - 1; i >= 0; --i) {
-      double v = (i == col) ? 1.0 : 0.0;
+      F v = (i == col) ? F(1.0) : F(0.0);
       for (int k = i + 1; k < B; ++k) v -= R[static_cast<size_t>(i) * B + k] * Rinv[static_cast<size_t>(k) * B + col];
       Rinv[static_cast<size_t>(i) * B + col] = v / R[static_cast<size_t>(i) * B + i];
     }
@@ -246,13 +300,21 @@ bool cholesky_upper(std::vector<double> const &G, int B, std::vector<double> &R,
 }
 
 // C = A * B for B x B row-major matrices.
-std::vector<double> matmul(std::vector<double> const &A, std::vector<double> const &Bm, int B) {
-  std::vector<double> C(static_cast<size_t>(B) * B, 0.0);
+template <typename F> std::vector<F> matmul(std::vector<F> const &A, std::vector<F> const &Bm, int B) {
+  std::vector<F> C(static_cast<size_t>(B) * B, F(0.0));
   for (int i = 0; i < B; ++i)
     for (int k = 0; k < B; ++k)
       for (int j = 0; j < B; ++j)
         C[static_cast<size_t>(i) * B + j] += A[static_cast<size_t>(i) * B + k] * Bm[static_cast<size_t>(k) * B + j];
   return C;
+}
+
+bool field_eigh(std::vector<double> const &T, int n, std::vector<double> &w, std::vector<double> &Z) {
+  return symmetric_eigh(T, n, w, Z);
+}
+bool field_eigh(std::vector<std::complex<double>> const &T, int n, std::vector<double> &w,
+                std::vector<std::complex<double>> &Z) {
+  return hermitian_eigh(T, n, w, Z);
 }
 
 template <typename F> void parallel_for(int count, F &&f) {
@@ -349,7 +411,11 @@ bool block_column(double const *X, int b, int col, double *out, int64_t rows) {
 
 bool check_block_krylov_options(BlockKrylovOptions const &o, int multiplicity) {
   int const B = o.block;
-  if (B != 1 && B != 2 && B != 4 && B != 8) return fail("block size must be 1, 2, 4 or 8");
+  if (o.complex_values) {
+    if (B != 1 && B != 2 && B != 4) return fail("block size must be 1, 2 or 4 complex columns");
+  } else if (B != 1 && B != 2 && B != 4 && B != 8) {
+    return fail("block size must be 1, 2, 4 or 8");
+  }
   if (o.nev < 1 || multiplicity * o.nev > B)
     return fail("nev (times 2 for complex sectors) must not exceed the block size");
   if (o.keep < 1 || o.keep >= o.max_blocks - 1) return fail("keep must be in [1, max_blocks - 2]");
@@ -357,13 +423,19 @@ bool check_block_krylov_options(BlockKrylovOptions const &o, int multiplicity) {
   return true;
 }
 
-bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const &active, int multiplicity,
-                        BlockProduct const &product, BlockKrylovOptions const &o, BlockKrylovResult &res,
-                        double *d_ritz_block) {
+namespace {
+
+// The solver over the field F (see block_krylov_solve).  Host-side matrices
+// are F; device blocks are real rows of Bw = Bf * width entries.
+template <typename F>
+bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const &active, int multiplicity,
+                  BlockProduct const &product, BlockKrylovOptions const &o, BlockKrylovResult &res,
+                  double *d_ritz_block) {
   auto const t_start = Clock::now();
   if (!check_block_krylov_options(o, multiplicity)) return false;
-  int const B = o.block;
-  int const want = multiplicity * o.nev; // Ritz values per segment
+  int const Bf = o.block;                        // columns of a block (field entries)
+  int const Bw = Bf * Field<F>::kWidth;          // reals per block row on the device
+  int const want = multiplicity * o.nev;         // Ritz values per segment
   int const nseg_all = static_cast<int>(seg_rows.size()) - 1;
   if (nseg_all < 1) return fail("no segments");
   if (!active.empty() && static_cast<int>(active.size()) != nseg_all) return fail("internal: bad segment mask");
@@ -374,16 +446,16 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
   int const ns = static_cast<int>(listed.size());
   if (ns < 1) return fail("no segments");
   int const m = o.max_blocks;
-  int const V = B * B;
+  int const Vf = Bf * Bf, Vw = Bw * Bw;
   int64_t min_rows = INT64_MAX;
   for (int s : listed) min_rows = std::min(min_rows, seg_rows[s + 1] - seg_rows[s]);
-  if (min_rows < static_cast<int64_t>(m + 1) * B)
+  if (min_rows < static_cast<int64_t>(m + 1) * Bf)
     return fail("a segment is too small for (max_blocks + 1) * block_size Krylov vectors (" +
                 std::to_string(min_rows) + " rows)");
   BlockAlgebra alg;
-  if (!alg.init(seg_rows, listed, B, m)) return false;
+  if (!alg.init(seg_rows, listed, Bw, m)) return false;
   int64_t const rows = alg.seg.rows;
-  int64_t const blk = rows * B; // doubles per block
+  int64_t const blk = rows * Bw; // reals per block
   // FP64 buffers; while the product is FP32 the blocks are stored as FP32 in
   // the same memory (block j at float offset j * blk).
   DeviceArray<double> Vb, W, Y;
@@ -412,28 +484,35 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
     ++prods;
     return true;
   };
-  std::vector<double> G, Rf(static_cast<size_t>(ns) * V), coef;
+  std::vector<double> G, coef;
+  std::vector<F> Gf, Rf(static_cast<size_t>(ns) * Vf);
+  // coef (real, per segment and block) from field matrices M[s][i]
+  auto set_coef = [&](std::vector<F> const &M, int nb) {
+    coef.assign(static_cast<size_t>(ns) * nb * Vw, 0.0);
+    for (size_t k = 0; k < static_cast<size_t>(ns) * nb; ++k)
+      field_to_coef(M.data() + k * Vf, Bf, coef.data() + k * Vw);
+  };
   // CholQR2 of block X into out (may be X); Rf receives the factor R per segment.
   auto cholqr = [&]<typename T>(T *X, T *out) -> bool {
-    std::vector<double> Racc(static_cast<size_t>(ns) * V, 0.0);
+    std::vector<F> Racc(static_cast<size_t>(ns) * Vf, F(0.0)), Minv(static_cast<size_t>(ns) * Vf);
     for (int s = 0; s < ns; ++s)
-      for (int i = 0; i < B; ++i) Racc[static_cast<size_t>(s) * V + i * B + i] = 1.0;
+      for (int i = 0; i < Bf; ++i) Racc[static_cast<size_t>(s) * Vf + i * Bf + i] = 1.0;
     T *src = X;
     for (int pass = 0; pass < 2; ++pass) {
       if (!alg.gram(src, blk, 1, src, G)) return false;
-      coef.assign(static_cast<size_t>(ns) * V, 0.0);
+      gram_to_field(G, static_cast<size_t>(ns), Bf, Gf);
       for (int s = 0; s < ns; ++s) {
-        std::vector<double> Gs(G.begin() + static_cast<int64_t>(s) * V, G.begin() + static_cast<int64_t>(s + 1) * V);
-        std::vector<double> R, Rinv;
-        if (!cholesky_upper(Gs, B, R, Rinv))
+        std::vector<F> Gs(Gf.begin() + static_cast<int64_t>(s) * Vf, Gf.begin() + static_cast<int64_t>(s + 1) * Vf);
+        std::vector<F> R, Rinv;
+        if (!cholesky_upper(Gs, Bf, R, Rinv))
           return fail("block Lanczos breakdown in segment " + std::to_string(listed[s]) +
                       " (the Krylov space is exhausted; lower max_blocks or the block size)");
-        std::copy(Rinv.begin(), Rinv.end(), coef.begin() + static_cast<int64_t>(s) * V);
-        std::vector<double> Ra(Racc.begin() + static_cast<int64_t>(s) * V,
-                               Racc.begin() + static_cast<int64_t>(s + 1) * V);
-        std::vector<double> const Rn = matmul(R, Ra, B);
-        std::copy(Rn.begin(), Rn.end(), Racc.begin() + static_cast<int64_t>(s) * V);
+        std::copy(Rinv.begin(), Rinv.end(), Minv.begin() + static_cast<int64_t>(s) * Vf);
+        std::vector<F> Ra(Racc.begin() + static_cast<int64_t>(s) * Vf, Racc.begin() + static_cast<int64_t>(s + 1) * Vf);
+        std::vector<F> const Rn = matmul(R, Ra, Bf);
+        std::copy(Rn.begin(), Rn.end(), Racc.begin() + static_cast<int64_t>(s) * Vf);
       }
+      set_coef(Minv, 1);
       if (!alg.combine(out, 0.0, src, blk, 1, coef)) return false;
       src = out;
     }
@@ -441,24 +520,27 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
     return true;
   };
 
-  int const dimT = m * B;
-  std::vector<std::vector<double>> Tp(ns, std::vector<double>(static_cast<size_t>(dimT) * dimT, 0.0));
-  std::vector<std::vector<double>> theta(ns), Z(ns);
+  int const dimT = m * Bf;
+  std::vector<std::vector<F>> Tp(ns, std::vector<F>(static_cast<size_t>(dimT) * dimT, F(0.0)));
+  std::vector<std::vector<double>> theta(ns);
+  std::vector<std::vector<F>> Z(ns);
   std::vector<double> worst(ns, 0.0);
   int kept = 0;
   int cycle = 0, cycles_run = 0, fp64_cycles = 0, blocks_done = 0;
   double prev_worst = INFINITY;
   bool done = false;
-  std::vector<double> C, Csum;
-  // coefficients of Ritz block t: coef[s][i] = Z_s[i-block rows, t-block columns]
+  std::vector<double> C;
+  std::vector<F> Cf, Csum, Mf;
+  // coefficients of Ritz block t: M[s][i] = Z_s[i-block rows, t-block columns]
   auto ritz_coef = [&](int t) {
-    coef.assign(static_cast<size_t>(ns) * m * V, 0.0);
+    Mf.assign(static_cast<size_t>(ns) * m * Vf, F(0.0));
     for (int s = 0; s < ns; ++s)
       for (int i = 0; i < m; ++i)
-        for (int p = 0; p < B; ++p)
-          for (int q = 0; q < B; ++q)
-            coef[(static_cast<size_t>(s) * m + i) * V + p * B + q] =
-                Z[s][static_cast<size_t>(i * B + p) * dimT + t * B + q];
+        for (int p = 0; p < Bf; ++p)
+          for (int q = 0; q < Bf; ++q)
+            Mf[(static_cast<size_t>(s) * m + i) * Vf + p * Bf + q] =
+                Z[s][static_cast<size_t>(i * Bf + p) * dimT + t * Bf + q];
+    set_coef(Mf, m);
   };
 
   // Restart cycles with the blocks stored as T: float runs the FP32 product,
@@ -475,28 +557,31 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
         if (!apply(fp32, Vj(j), Wt)) return -1;
         ++blocks_done;
         int const nb = j + 1;
-        Csum.assign(static_cast<size_t>(ns) * nb * V, 0.0);
+        Csum.assign(static_cast<size_t>(ns) * nb * Vf, F(0.0));
         for (int pass = 0; pass < 2; ++pass) {
           if (!alg.gram(Vbase, blk, nb, Wt, C)) return -1;
-          for (size_t t = 0; t < C.size(); ++t) {
-            Csum[t] += C[t];
-            C[t] = -C[t];
+          gram_to_field(C, static_cast<size_t>(ns) * nb, Bf, Cf);
+          for (size_t t = 0; t < Cf.size(); ++t) {
+            Csum[t] += Cf[t];
+            Cf[t] = -Cf[t];
           }
-          if (!alg.combine(Wt, 1.0, Vbase, blk, nb, C)) return -1;
+          set_coef(Cf, nb);
+          if (!alg.combine(Wt, 1.0, Vbase, blk, nb, coef)) return -1;
         }
-        // column j of the projected matrix (and its mirror) from the coefficients
+        // column j of the projected matrix (and its mirror) from the
+        // coefficients V_i^H A V_j
         for (int s = 0; s < ns; ++s) {
-          std::vector<double> &Ts = Tp[s];
+          std::vector<F> &Ts = Tp[s];
           for (int i = 0; i < nb; ++i) {
-            double const *Ci = Csum.data() + (static_cast<size_t>(s) * nb + i) * V;
-            for (int p = 0; p < B; ++p)
-              for (int q = 0; q < B; ++q) {
-                size_t const r = static_cast<size_t>(i * B + p), c = static_cast<size_t>(j * B + q);
+            F const *Ci = Csum.data() + (static_cast<size_t>(s) * nb + i) * Vf;
+            for (int p = 0; p < Bf; ++p)
+              for (int q = 0; q < Bf; ++q) {
+                size_t const r = static_cast<size_t>(i * Bf + p), c = static_cast<size_t>(j * Bf + q);
                 if (i == j) {
-                  Ts[r * dimT + c] = 0.5 * (Ci[p * B + q] + Ci[q * B + p]);
+                  Ts[r * dimT + c] = 0.5 * (Ci[p * Bf + q] + Field<F>::conj(Ci[q * Bf + p]));
                 } else {
-                  Ts[r * dimT + c] = Ci[p * B + q];
-                  Ts[c * dimT + r] = Ci[p * B + q];
+                  Ts[r * dimT + c] = Ci[p * Bf + q];
+                  Ts[c * dimT + r] = Field<F>::conj(Ci[p * Bf + q]);
                 }
               }
           }
@@ -510,16 +595,17 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
       // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
       std::vector<int> eig_ok(ns, 1);
       parallel_for(ns, [&](int s) {
-        eig_ok[s] = symmetric_eigh(Tp[s], dimT, theta[s], Z[s]) ? 1 : 0;
+        eig_ok[s] = field_eigh(Tp[s], dimT, theta[s], Z[s]) ? 1 : 0;
         if (!eig_ok[s]) return;
         double w = 0.0;
-        double const *Rs = Rf.data() + static_cast<size_t>(s) * V;
+        F const *Rs = Rf.data() + static_cast<size_t>(s) * Vf;
         for (int q = 0; q < want; ++q) {
           double rs = 0.0;
-          for (int p = 0; p < B; ++p) {
-            double acc = 0.0;
-            for (int c = 0; c < B; ++c) acc += Rs[p * B + c] * Z[s][static_cast<size_t>((m - 1) * B + c) * dimT + q];
-            rs += acc * acc;
+          for (int p = 0; p < Bf; ++p) {
+            F acc = 0.0;
+            for (int c = 0; c < Bf; ++c)
+              acc += Rs[p * Bf + c] * Z[s][static_cast<size_t>((m - 1) * Bf + c) * dimT + q];
+            rs += std::norm(acc);
           }
           w = std::max(w, std::sqrt(rs) / std::max(1.0, std::fabs(theta[s][q])));
         }
@@ -552,8 +638,8 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
       LSAMD_HIP(hipMemcpyAsync(Vbase, Yt, sizeof(T) * blk * r, hipMemcpyDeviceToDevice, stream()));
       LSAMD_HIP(hipMemcpyAsync(Vj(r), Wt, sizeof(T) * blk, hipMemcpyDeviceToDevice, stream()));
       for (int s = 0; s < ns; ++s) {
-        std::fill(Tp[s].begin(), Tp[s].end(), 0.0);
-        for (int q = 0; q < r * B; ++q) Tp[s][static_cast<size_t>(q) * dimT + q] = theta[s][q];
+        std::fill(Tp[s].begin(), Tp[s].end(), F(0.0));
+        for (int q = 0; q < r * Bf; ++q) Tp[s][static_cast<size_t>(q) * dimT + q] = theta[s][q];
       }
       kept = r;
       LSAMD_HIP(hipStreamSynchronize(stream()));
@@ -575,7 +661,7 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
       LSAMD_HIP(hipStreamSynchronize(stream()));
       if (!clear_unsolved(false)) return false;
       LSAMD_HIP(hipMemcpyAsync(Vb.ptr, Y.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
-      for (int s = 0; s < ns; ++s) std::fill(Tp[s].begin(), Tp[s].end(), 0.0);
+      for (int s = 0; s < ns; ++s) std::fill(Tp[s].begin(), Tp[s].end(), F(0.0));
       kept = 0;
       if (!cholqr(Vb.ptr, Vb.ptr)) return false;
     }
@@ -588,12 +674,16 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
   if (status == 1 && run_cycles(Vb.ptr, W.ptr, Y.ptr) < 0) return false;
   // Final: the lowest Ritz block (in Y), its FP64 Rayleigh quotients and residuals.
   if (!apply(false, Y.ptr, W.ptr)) return false;
-  std::vector<double> Th, Gr;
-  if (!alg.gram(Y.ptr, blk, 1, W.ptr, Th)) return false; // Y^T A Y
-  std::vector<double> negTh(Th.size());
+  std::vector<double> Gr;
+  std::vector<F> Th, Grf;
+  if (!alg.gram(Y.ptr, blk, 1, W.ptr, G)) return false; // Y^H A Y
+  gram_to_field(G, static_cast<size_t>(ns), Bf, Th);
+  std::vector<F> negTh(Th.size());
   for (size_t t = 0; t < Th.size(); ++t) negTh[t] = -Th[t];
-  if (!alg.combine(W.ptr, 1.0, Y.ptr, blk, 1, negTh)) return false; // A Y - Y (Y^T A Y)
+  set_coef(negTh, 1);
+  if (!alg.combine(W.ptr, 1.0, Y.ptr, blk, 1, coef)) return false; // A Y - Y (Y^H A Y)
   if (!alg.gram(W.ptr, blk, 1, W.ptr, Gr)) return false;
+  gram_to_field(Gr, static_cast<size_t>(ns), Bf, Grf);
   res.evals.assign(static_cast<size_t>(nseg_all) * o.nev, NAN);
   res.residuals.assign(static_cast<size_t>(nseg_all) * o.nev, NAN);
   double rmax = 0.0;
@@ -602,8 +692,8 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
       double lam = 0.0, r2 = 0.0;
       for (int c = 0; c < multiplicity; ++c) {
         int const q = multiplicity * e + c;
-        lam += Th[static_cast<size_t>(s) * V + q * B + q];
-        r2 = std::max(r2, Gr[static_cast<size_t>(s) * V + q * B + q]);
+        lam += Field<F>::re(Th[static_cast<size_t>(s) * Vf + q * Bf + q]);
+        r2 = std::max(r2, Field<F>::re(Grf[static_cast<size_t>(s) * Vf + q * Bf + q]));
       }
       lam /= multiplicity;
       double const rel = std::sqrt(std::max(0.0, r2)) / std::max(1.0, std::fabs(lam));
@@ -611,8 +701,8 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
       res.residuals[static_cast<size_t>(listed[s]) * o.nev + e] = rel;
       rmax = std::max(rmax, rel);
     }
-  res.first_rayleigh = Th[0];
-  res.first_residual = std::sqrt(std::max(0.0, Gr[0]));
+  res.first_rayleigh = Field<F>::re(Th[0]);
+  res.first_residual = std::sqrt(std::max(0.0, Field<F>::re(Grf[0])));
   if (d_ritz_block != nullptr)
     LSAMD_HIP(hipMemcpyAsync(d_ritz_block, Y.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
   LSAMD_HIP(hipStreamSynchronize(stream()));
@@ -625,6 +715,16 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
   res.max_residual = rmax;
   res.total_seconds = seconds_since(t_start);
   return true;
+}
+
+} // namespace
+
+bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const &active, int multiplicity,
+                        BlockProduct const &product, BlockKrylovOptions const &o, BlockKrylovResult &res,
+                        double *d_ritz_block) {
+  if (o.complex_values)
+    return krylov_solve<std::complex<double>>(seg_rows, active, multiplicity, product, o, res, d_ritz_block);
+  return krylov_solve<double>(seg_rows, active, multiplicity, product, o, res, d_ritz_block);
 }
 
 } // namespace lsamd

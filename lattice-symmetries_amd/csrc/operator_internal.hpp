@@ -1,6 +1,7 @@
 // operator_internal.hpp -- operator objects shared by operator.hip and abi.cpp.
 #pragma once
 
+#include <complex>
 #include <functional>
 
 #include "lsamd_internal.hpp"
@@ -105,6 +106,9 @@ struct BlockKrylovOptions {
   double switch_tol = 1e-5; // relative residual at which FP32 gives way to FP64
   uint64_t seed = 1;
   bool fp32 = true;         // start with the FP32 product
+  // blocks of `block` complex columns, each row stored as [re | im] (2 * block
+  // reals), with complex (Hermitian) arithmetic on the host side
+  bool complex_values = false;
 };
 
 struct BlockKrylovResult {
@@ -135,6 +139,9 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
 // Dense symmetric eigen-decomposition (dense_eigh.cpp): A is n x n row-major;
 // w receives the eigenvalues ascending and Z the eigenvectors as columns.
 bool symmetric_eigh(std::vector<double> A, int n, std::vector<double> &w, std::vector<double> &Z);
+// Complex Hermitian eigen-decomposition (cyclic Jacobi), same conventions.
+bool hermitian_eigh(std::vector<std::complex<double>> A, int n, std::vector<double> &w,
+                    std::vector<std::complex<double>> &Z);
 bool block_column(double const *X, int b, int col, double *out, int64_t rows);
 
 } // namespace lsamd

@@ -5,16 +5,15 @@
 // spin-inversion eigenvalues) each give a complex Hermitian matrix
 // M_s = A_s + i B_s.  Their complex CSR caches are concatenated into one
 // block-diagonal CSR with 32-bit global columns.  Vectors carry K complex
-// right-hand sides per state, laid out [state][re | im][K]; read as
-// [2 * state + part][K] this is the real embedding [[A, -B], [B, A]] of M,
-// whose spectrum is that of M with every eigenvalue twice.
+// right-hand sides per state, laid out [state][re | im][K].
 //
 // The product is one persistent launch over every sector (a fixed grid of
 // workgroups looping over row tiles, optionally with each accelerator die's
 // workgroups kept on one contiguous range of rows), captured once in a HIP
 // graph and replayed.  The eigensolver is the segmented thick-restart block
-// Lanczos of block_krylov.hip with one segment per sector: every block step
-// is one product launch over all sectors.
+// Lanczos of block_krylov.hip in complex arithmetic, a block being K complex
+// columns in the same [re | im] layout, with one segment per sector: every
+// block step is one product launch over all sectors.
 //
 // Reference counterpart: the reference builds one basis per sector and runs
 // PRIMME / scipy eigsh on each in turn (chapel/src/Diagonalize.chpl:134-162,
@@ -457,16 +456,18 @@ bool dense_sector(ls_amd_sector_batch const &bt, int s, int nev, double *evals, 
   return true;
 }
 
-// The sectors' rows in the real embedding are [2 offsets[s], 2 offsets[s+1]);
-// each eigenvalue of a sector appears twice there (multiplicity 2).  Sectors
-// with fewer states than the Krylov space, (max_blocks + 1) * block, are
-// diagonalised densely instead and left out of the block Lanczos process.
-bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &o, BlockKrylovResult &res) {
+// Complex block Lanczos over all sectors: a block is `block` complex columns,
+// stored per state as [re | im] (the product's layout), and sector s owns
+// rows [offsets[s], offsets[s+1]).  Sectors with fewer states than twice the
+// Krylov space, 2 (max_blocks + 1) * block, are diagonalised densely instead
+// and left out of the block Lanczos process.
+bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &opts, BlockKrylovResult &res) {
   auto const t_start = std::chrono::steady_clock::now();
+  BlockKrylovOptions o = opts;
+  o.complex_values = true;
   int const ns = bt.count;
-  int64_t const krylov_min = static_cast<int64_t>(o.max_blocks + 1) * o.block;
-  std::vector<int64_t> seg_rows(bt.offsets.size());
-  for (size_t s = 0; s < seg_rows.size(); ++s) seg_rows[s] = 2 * bt.offsets[s];
+  int64_t const krylov_min = 2 * static_cast<int64_t>(o.max_blocks + 1) * o.block;
+  std::vector<int64_t> const &seg_rows = bt.offsets;
   std::vector<char> active(ns, 1);
   int dense = 0;
   for (int s = 0; s < ns; ++s) {
@@ -487,10 +488,10 @@ bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &o, Blo
       LSAMD_HIP(hipGetLastError());
       return true;
     };
-    if (!block_krylov_solve(seg_rows, active, 2, prod, o, res, nullptr)) return false;
+    if (!block_krylov_solve(seg_rows, active, 1, prod, o, res, nullptr)) return false;
   } else {
     // nothing for the Krylov process; the options are still checked alike
-    if (!check_block_krylov_options(o, 2)) return false;
+    if (!check_block_krylov_options(o, 1)) return false;
     res = BlockKrylovResult{};
     res.converged = true;
     res.evals.assign(static_cast<size_t>(ns) * o.nev, NAN);

@@ -241,6 +241,7 @@ SIGNATURES = {
     "ls_amd_block_lanczos": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int,
                                        _P, C.POINTER(ls_amd_lanczos_result)]),
     "ls_amd_dense_eigh": (C.c_int, [C.c_int, _f64p, _f64p, _f64p]),
+    "ls_amd_dense_eigh_hermitian": (C.c_int, [C.c_int, _f64p, _f64p, _f64p]),
     "ls_amd_gather_probe": (C.c_int, [C.c_int64, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, _f64p]),
     "ls_amd_model_from_yaml": (_P, [C.c_char_p]),
     "ls_amd_model_destroy": (None, [_P]),

@@ -91,7 +91,7 @@ def test_batch_mixing_large_small_and_empty_sectors(engine):
     batch = lsamd.DeviceSectorBatch(ops)
     rng = np.random.default_rng(5)
     check_batch_product(batch, mats, rng, ks=(2, 8))
-    r = batch.eigensolve(nev=4, block_size=8, max_blocks=10, keep_blocks=2, max_cycles=40, tol=1e-8)
+    r = batch.eigensolve(nev=4, block_size=4, max_blocks=10, keep_blocks=2, max_cycles=40, tol=1e-8)
     say("cycles", r["cycles"], "fp64", r["fp64_cycles"], "resid", r["max_residual"])
     assert r["converged"] and r["products"] > 0
     check_spectra(r, 4, spectra)

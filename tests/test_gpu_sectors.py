@@ -103,7 +103,7 @@ def test_sector_eigensolve_matches_dense(engine):
     say("sector eigensolve", n)
     batch = LS.chain_sector_batch(n, h, expr)
     ref = oracle_sector_matrices(n, h, expr, batch.sectors)
-    r = batch.eigensolve(nev=4, block_size=8, max_blocks=10, keep_blocks=2, max_cycles=40, tol=1e-8)
+    r = batch.eigensolve(nev=4, block_size=4, max_blocks=10, keep_blocks=2, max_cycles=40, tol=1e-8)
     say("cycles", r["cycles"], "fp64", r["fp64_cycles"], "products", r["products"], "resid", r["max_residual"])
     assert r["converged"]
     assert r["fp64_cycles"] >= 1
@@ -190,7 +190,7 @@ def test_chain28_sector_batch_full_size(engine):
     refs = {nm: ops[names.index(nm)] for nm in ("5+", "11-")}
     del bases, ops
     batch.autotune(8)
-    r = batch.eigensolve(nev=4, block_size=8, max_blocks=12, keep_blocks=3, max_cycles=40, tol=1e-7)
+    r = batch.eigensolve(nev=4, block_size=4, max_blocks=16, keep_blocks=4, max_cycles=40, tol=1e-7)
     say("cycles", r["cycles"], "fp64", r["fp64_cycles"], "seconds", r["total_seconds"])
     assert r["converged"] and r["max_residual"] < 1e-6
     ev = r["eigenvalues"]
