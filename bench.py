@@ -130,7 +130,7 @@ def sectors_bench(args):
     value = 8.0 * nnz * args.steps / t_steps
     del dx, dy
     r = batch.eigensolve(nev=4, block_size=4, max_blocks=args.sector_blocks, keep_blocks=args.sector_keep,
-                         max_cycles=args.sector_cycles, tol=1e-7, switch_tol=1e-5, seed=1)
+                         max_cycles=args.sector_cycles, tol=1e-7, switch_tol=args.sector_switch_tol, seed=1)
     log(f"eigensolve: {r['cycles']} cycles ({r['fp64_cycles']} FP64), {r['products']} products, "
         f"{r['total_seconds']:.1f} s, max residual {r['max_residual']:.2e}")
     ev = r["eigenvalues"]
@@ -167,7 +167,7 @@ def sectors_bench(args):
         "setup_seconds": t_setup,
         "eigensolve": {"method": "complex thick-restart block Lanczos, FP32 then FP64",
                        "block_complex_columns": 4, "max_blocks": args.sector_blocks,
-                       "keep_blocks": args.sector_keep,
+                       "keep_blocks": args.sector_keep, "switch_tol": args.sector_switch_tol,
                        "cycles": r["cycles"], "fp64_cycles": r["fp64_cycles"], "products": r["products"],
                        "product_seconds": r["product_seconds"], "seconds": r["total_seconds"],
                        "max_residual": r["max_residual"], "converged": r["converged"],
@@ -240,6 +240,8 @@ def main():
     ap.add_argument("--sector-blocks", type=int, default=16)
     ap.add_argument("--sector-cycles", type=int, default=40)
     ap.add_argument("--sector-keep", type=int, default=4)
+    ap.add_argument("--sector-switch-tol", type=float, default=1e-5,
+                    help="relative residual at which the sector eigensolver leaves the FP32 product")
     ap.add_argument("--reference-sectors", default="0+,14+,7-,13+")
     ap.add_argument("--reference-steps", type=int, default=400)
     args = ap.parse_args()

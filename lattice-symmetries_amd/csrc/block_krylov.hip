@@ -103,15 +103,29 @@ __global__ __launch_bounds__(kThreads) void seg_gram_kernel(TA const *__restrict
   }
 }
 
-// out[(s * nblk + i) * V + t] = sum over the chunks of segment s, in order.
-__global__ __launch_bounds__(64) void seg_reduce_kernel(double const *__restrict__ partial, int nblk, int V,
-                                                        int const *__restrict__ seg_chunk,
-                                                        double *__restrict__ out) {
+// out[(s * nblk + i) * V + v] = sum over the chunks of segment s.  The
+// kReduceThreads threads form G = kReduceThreads / V groups; group g sums
+// chunks c0 + g, c0 + g + G, ... in order, and the G group sums are added in
+// group order, so the result does not depend on scheduling.
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void seg_reduce_kernel(double const *__restrict__ partial, int nblk,
+                                                                    int V, int const *__restrict__ seg_chunk,
+                                                                    double *__restrict__ out) {
+  __shared__ double part[kReduceThreads];
   int const s = blockIdx.x, i = blockIdx.y, t = threadIdx.x;
-  if (t >= V) return;
+  int const G = kReduceThreads / V, v = t % V, g = t / V;
   double sum = 0.0;
-  for (int c = seg_chunk[s]; c < seg_chunk[s + 1]; ++c) sum += partial[(static_cast<int64_t>(c) * nblk + i) * V + t];
-  out[(static_cast<int64_t>(s) * nblk + i) * V + t] = sum;
+  if (g < G)
+    for (int c = seg_chunk[s] + g; c < seg_chunk[s + 1]; c += G)
+      sum += partial[(static_cast<int64_t>(c) * nblk + i) * V + v];
+  part[t] = sum;
+  __syncthreads();
+  if (t < V) {
+    double tot = 0.0;
+    for (int h = 0; h < G; ++h) tot += part[h * V + t];
+    out[(static_cast<int64_t>(s) * nblk + i) * V + t] = tot;
+  }
 }
 
 // out[r][q] = beta * out[r][q] + sum_{i < nblk} sum_p A_i[r][p] C_s[i][p][q]
@@ -359,8 +373,8 @@ struct BlockAlgebra {
         return true;
       });
       if (!ok) return false;
-      seg_reduce_kernel<<<dim3(seg.nseg, nblk), 64, 0, stream()>>>(partial.ptr, nblk, V, seg.d_seg_chunk.ptr,
-                                                                    d_out.ptr);
+      seg_reduce_kernel<<<dim3(seg.nseg, nblk), kReduceThreads, 0, stream()>>>(partial.ptr, nblk, V,
+                                                                                seg.d_seg_chunk.ptr, d_out.ptr);
       LSAMD_HIP(hipGetLastError());
     }
     out_host.assign(static_cast<size_t>(seg.nseg) * nblk * V, 0.0);
