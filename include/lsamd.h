@@ -393,8 +393,9 @@ int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size,
 int ls_amd_dense_eigh(int n, double const *A, double *w, double *Z);
 
 /* Host utility: the same for a dense complex Hermitian n x n matrix A
-   (row-major, interleaved re, im), by cyclic Jacobi with complex rotations,
-   as the block solver uses for the projected matrices of complex sectors.
+   (row-major, interleaved re, im), by complex Householder tridiagonalisation
+   and the same implicit QR, as the block solver uses for the projected
+   matrices of complex sectors.
    w[n] ascending; Z (may be NULL) the eigenvectors as columns, interleaved. */
 int ls_amd_dense_eigh_hermitian(int n, double const *A, double *w, double *Z);
 

@@ -7,9 +7,11 @@
 // 4th ed., algorithms 8.3.1 and 8.3.2), with eigenvectors carried along.
 // About 8 n^3 flops, against roughly 10 sweeps x 6 n^3 for cyclic Jacobi.
 //
-// Complex Hermitian matrices (the projected matrices of complex sectors) go
-// through cyclic Jacobi with complex rotations (Golub & Van Loan, 4th ed.,
-// 8.5.2, after a diagonal phase that makes the pivot element real).
+// Complex Hermitian matrices (the projected matrices of complex sectors) are
+// reduced the same way with complex Householder reflections (H = I - beta v
+// v^H, H x = -e^{i arg x_0} |x| e_1); a diagonal phase D makes the
+// tridiagonal real, the same implicit QR follows, and the eigenvectors are
+// Q D Z.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -158,63 +160,94 @@ bool hermitian_eigh(std::vector<std::complex<double>> A, int n, std::vector<doub
     return true;
   }
   auto a = [&](int i, int j) -> C & { return A[static_cast<size_t>(i) * n + j]; };
-  auto z = [&](int i, int j) -> C & { return Z[static_cast<size_t>(i) * n + j]; };
-  Z.assign(static_cast<size_t>(n) * n, C(0.0));
-  for (int i = 0; i < n; ++i) {
-    z(i, i) = 1.0;
-    a(i, i) = a(i, i).real();
-  }
-  bool converged = false;
-  for (int sweep = 0; sweep < 60 && !converged; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int p = 0; p < n; ++p) {
-      diag += std::norm(a(p, p));
-      for (int q = p + 1; q < n; ++q) off += std::norm(a(p, q));
+  // Householder reduction A = Q T Q^H, T Hermitian tridiagonal
+  std::vector<C> Q(static_cast<size_t>(n) * n, C(0.0));
+  for (int i = 0; i < n; ++i) Q[static_cast<size_t>(i) * n + i] = 1.0;
+  std::vector<C> v(n), p(n), q(n), qv(n);
+  for (int k = 0; k + 2 < n; ++k) {
+    int const len = n - k - 1;
+    double norm2 = 0.0;
+    for (int i = 0; i < len; ++i) norm2 += std::norm(a(k + 1 + i, k));
+    C const x0 = a(k + 1, k);
+    if (norm2 - std::norm(x0) == 0.0) continue; // column already reduced
+    double const norm = std::sqrt(norm2), ax0 = std::abs(x0);
+    C const phase = ax0 > 0.0 ? x0 / ax0 : C(1.0);
+    C const alpha = -phase * norm; // H x = alpha e_1
+    for (int i = 0; i < len; ++i) v[i] = a(k + 1 + i, k);
+    v[0] -= alpha;
+    double vn2 = 0.0;
+    for (int i = 0; i < len; ++i) vn2 += std::norm(v[i]);
+    double const beta = 2.0 / vn2;
+    // S <- H S H on the trailing block S = A[k+1:, k+1:], H = I - beta v v^H
+    for (int i = 0; i < len; ++i) {
+      C s = 0.0;
+      for (int j = 0; j < len; ++j) s += a(k + 1 + i, k + 1 + j) * v[j];
+      p[i] = beta * s;
     }
-    if (off <= 1e-32 * (diag + 2.0 * off)) {
-      converged = true;
-      break;
+    C vp = 0.0;
+    for (int i = 0; i < len; ++i) vp += std::conj(v[i]) * p[i];
+    double const K = 0.5 * beta * vp.real();
+    for (int i = 0; i < len; ++i) q[i] = p[i] - K * v[i];
+    for (int i = 0; i < len; ++i)
+      for (int j = 0; j < len; ++j)
+        a(k + 1 + i, k + 1 + j) -= v[i] * std::conj(q[j]) + q[i] * std::conj(v[j]);
+    a(k + 1, k) = alpha;
+    a(k, k + 1) = std::conj(alpha);
+    for (int i = 1; i < len; ++i) a(k + 1 + i, k) = a(k, k + 1 + i) = 0.0;
+    // Q <- Q H on columns k+1 ..
+    for (int r = 0; r < n; ++r) {
+      C s = 0.0;
+      for (int j = 0; j < len; ++j) s += Q[static_cast<size_t>(r) * n + k + 1 + j] * v[j];
+      qv[r] = beta * s;
     }
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        double const r = std::abs(a(p, q));
-        if (r == 0.0) continue;
-        // U = diag(1, e) [[c, s], [-s, c]] on (p, q), e = conj(a_pq) / |a_pq|
-        C const e = std::conj(a(p, q)) / r;
-        double const tau = (a(q, q).real() - a(p, p).real()) / (2.0 * r);
-        double const t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
-        double const c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
-        for (int k = 0; k < n; ++k) { // A <- A U
-          C const kp = a(k, p), kq = a(k, q);
-          a(k, p) = c * kp - s * e * kq;
-          a(k, q) = s * kp + c * e * kq;
-        }
-        for (int k = 0; k < n; ++k) { // A <- U^H A
-          C const pk = a(p, k), qk = a(q, k);
-          a(p, k) = c * pk - s * std::conj(e) * qk;
-          a(q, k) = s * pk + c * std::conj(e) * qk;
-        }
-        a(p, q) = a(q, p) = 0.0;
-        a(p, p) = a(p, p).real();
-        a(q, q) = a(q, q).real();
-        for (int k = 0; k < n; ++k) { // Z <- Z U
-          C const kp = z(k, p), kq = z(k, q);
-          z(k, p) = c * kp - s * e * kq;
-          z(k, q) = s * kp + c * e * kq;
-        }
-      }
+    for (int r = 0; r < n; ++r)
+      for (int j = 0; j < len; ++j) Q[static_cast<size_t>(r) * n + k + 1 + j] -= qv[r] * std::conj(v[j]);
   }
-  if (!converged) return false;
+  // D^H T D is real with D = diag(ph), ph_{i+1} = ph_i t_i / |t_i| for the
+  // subdiagonal t_i; then the real implicit QR on (d, |t|)
+  std::vector<double> d(n), e(std::max(n - 1, 0));
+  std::vector<C> ph(n, C(1.0));
+  for (int i = 0; i < n; ++i) d[i] = a(i, i).real();
+  for (int i = 0; i + 1 < n; ++i) {
+    C const t = a(i + 1, i);
+    double const at = std::abs(t);
+    e[i] = at;
+    ph[i + 1] = at > 0.0 ? ph[i] * (t / at) : ph[i];
+  }
+  std::vector<double> Zr(static_cast<size_t>(n) * n, 0.0);
+  for (int i = 0; i < n; ++i) Zr[static_cast<size_t>(i) * n + i] = 1.0;
+  double const eps = std::numeric_limits<double>::epsilon();
+  int hi = n - 1;
+  int iterations = 0;
+  while (hi > 0) {
+    if (std::fabs(e[hi - 1]) <= eps * (std::fabs(d[hi - 1]) + std::fabs(d[hi]))) {
+      e[hi - 1] = 0.0;
+      --hi;
+      continue;
+    }
+    int lo = hi - 1;
+    while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(d[lo - 1]) + std::fabs(d[lo]))) --lo;
+    if (++iterations > 60 * n) return false;
+    qr_step(d, e, lo, hi, Zr, n);
+  }
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](int i, int j) { return a(i, i).real() < a(j, j).real(); });
+  std::sort(order.begin(), order.end(), [&](int i, int j) { return d[i] < d[j]; });
   w.resize(n);
-  std::vector<C> Zs(static_cast<size_t>(n) * n);
+  // eigenvectors Q D Zr, columns in ascending order
+  std::vector<C> QD(static_cast<size_t>(n) * n);
+  for (int r = 0; r < n; ++r)
+    for (int j = 0; j < n; ++j) QD[static_cast<size_t>(r) * n + j] = Q[static_cast<size_t>(r) * n + j] * ph[j];
+  Z.assign(static_cast<size_t>(n) * n, C(0.0));
   for (int c = 0; c < n; ++c) {
-    w[c] = a(order[c], order[c]).real();
-    for (int r = 0; r < n; ++r) Zs[static_cast<size_t>(r) * n + c] = z(r, order[c]);
+    int const oc = order[c];
+    w[c] = d[oc];
+    for (int r = 0; r < n; ++r) {
+      C s = 0.0;
+      for (int j = 0; j < n; ++j) s += QD[static_cast<size_t>(r) * n + j] * Zr[static_cast<size_t>(j) * n + oc];
+      Z[static_cast<size_t>(r) * n + c] = s;
+    }
   }
-  Z.swap(Zs);
   return true;
 }
 
@@ -229,7 +262,7 @@ extern "C" int ls_amd_dense_eigh_hermitian(int n, double const *A, double *w, do
   for (size_t i = 0; i < nn; ++i) a[i] = {A[2 * i], A[2 * i + 1]};
   std::vector<double> ww;
   if (!hermitian_eigh(std::move(a), n, ww, zz))
-    return fail("ls_amd_dense_eigh_hermitian: Jacobi sweeps did not converge") ? 0 : -1;
+    return fail("ls_amd_dense_eigh_hermitian: QR iteration did not converge") ? 0 : -1;
   std::copy(ww.begin(), ww.end(), w);
   if (Z != nullptr)
     for (size_t i = 0; i < nn; ++i) {

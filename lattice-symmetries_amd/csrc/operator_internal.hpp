@@ -139,7 +139,7 @@ bool block_krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> 
 // Dense symmetric eigen-decomposition (dense_eigh.cpp): A is n x n row-major;
 // w receives the eigenvalues ascending and Z the eigenvectors as columns.
 bool symmetric_eigh(std::vector<double> A, int n, std::vector<double> &w, std::vector<double> &Z);
-// Complex Hermitian eigen-decomposition (cyclic Jacobi), same conventions.
+// Complex Hermitian eigen-decomposition (complex Householder + the same QR), same conventions.
 bool hermitian_eigh(std::vector<std::complex<double>> A, int n, std::vector<double> &w,
                     std::vector<std::complex<double>> &Z);
 bool block_column(double const *X, int b, int col, double *out, int64_t rows);

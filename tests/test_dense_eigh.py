@@ -96,7 +96,7 @@ def check_hermitian(H, tol=1e-12):
 
 @pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 96])
 def test_random_hermitian(n):
-    """ls_amd_dense_eigh_hermitian (complex Jacobi) against numpy."""
+    """ls_amd_dense_eigh_hermitian (complex Householder + implicit QR) against numpy."""
     rng = np.random.default_rng(100 + n)
     M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     check_hermitian(M + M.conj().T)
