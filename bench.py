@@ -34,6 +34,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def traffic_rate(traffic, kernel_ms):
+    """The measured HBM bytes per launch over the measured kernel time: the
+    rate the memory system actually sustains, beside the algorithmic one."""
+    if traffic is None or not kernel_ms:
+        return {"traffic_GBs": None, "traffic_frac": None}
+    gbs = traffic / (kernel_ms * 1e-3) / 1e9
+    return {"traffic_GBs": gbs, "traffic_frac": gbs / HBM_PEAK_GBS}
+
+
 def profiled_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy of known size), as recorded in
@@ -163,6 +172,7 @@ def sectors_bench(args):
                    "states": dim, "off_diagonal_nonzeros": nnz_off, "schedule": sched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     **traffic_rate(traffic, ms),
                      "kernel": sector_kernel, "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch},
         "setup_seconds": t_setup,
         "eigensolve": {"method": "complex thick-restart block Lanczos, FP32 then FP64",
@@ -374,6 +384,7 @@ def main():
                    "schedule": sched, "value_table": csr_info["table_size"] if packed else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     **traffic_rate(traffic, k_ms),
                      "kernel": kernel_name, "kernel_ms": k_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "basis_build": {"states": dim, "candidates": candidates, "seconds": t_build,
