@@ -299,6 +299,8 @@ def main():
     packed = csr_info["packed"]
     tf = lambda b: "true" if b else "false"  # noqa: E731
     kernel_name = f"csr_f64_kernel<{lanes}, {tf(nt)}, {tf(die)}, {tf(packed)}>"
+    if die and sched["flags_f64"] & 4:
+        kernel_name += " full grid"
     log(f"csr: value table {'of ' + str(csr_info['table_size']) + ' values' if packed else 'not used'}")
 
     rng = np.random.default_rng(42)

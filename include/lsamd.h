@@ -314,8 +314,9 @@ int ls_amd_operator_autotune(ls_amd_operator *op);
 /* Current CSR product schedule: out[5] = lanes per row (FP64), flags (FP64),
    lanes per row (FP32 block), flags (FP32 block), persistent grid size.
    flags: bit 0 non-temporal loads of the CSR stream, bit 1 die-aware
-   persistent grid (each accelerator die's workgroups walk one contiguous
-   range of rows). */
+   mapping (each accelerator die's workgroups walk one contiguous range of
+   rows) on the persistent grid, bit 2 (with bit 1) the die-aware mapping
+   over one workgroup per tile instead. */
 int ls_amd_operator_schedule(ls_amd_operator const *op, int *out);
 int ls_amd_operator_set_schedule(ls_amd_operator *op, int lanes_f64,
                                  int flags_f64, int lanes_f32, int flags_f32);

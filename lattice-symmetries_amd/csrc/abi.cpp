@@ -809,7 +809,8 @@ extern "C" int ls_amd_operator_set_schedule(ls_amd_operator *op, int lanes_f64, 
   if (op == nullptr) return fail("null argument") ? 0 : -1;
   for (int l : {lanes_f64, lanes_f32})
     if (l < 1 || l > 64 || (l & (l - 1)) != 0) return fail("lanes per row must be a power of two <= 64") ? 0 : -1;
-  if (flags_f64 < 0 || flags_f64 > 3 || flags_f32 < 0 || flags_f32 > 3) return fail("schedule flags are 0..3") ? 0 : -1;
+  for (int f : {flags_f64, flags_f32})
+    if (f < 0 || f > 7 || (f == 4 || f == 5)) return fail("schedule flags are 0..3, 6 or 7") ? 0 : -1;
   op->core.lanes_f64 = lanes_f64;
   op->core.flags_f64 = flags_f64;
   op->core.lanes_f32 = lanes_f32;

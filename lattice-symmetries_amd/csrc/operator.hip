@@ -883,10 +883,14 @@ template <typename F> bool dispatch_flags(int flags, F &&f) {
   }
 }
 
-// Grid of a product launch: one workgroup per tile, or the persistent grid.
+// Grid of a product launch: one workgroup per tile; with the die-aware
+// mapping (flags bit 1) either the persistent grid or, with bit 2, one
+// workgroup per tile rounded up to a multiple of 8, where each workgroup
+// takes one tile of its die's contiguous range.
 int product_grid(OperatorCore const &op, int64_t n, int lpr, int flags) {
-  if (flags & 2) return op.persistent_blocks;
-  return grid_for(n * lpr, 256);
+  int const tiles = grid_for(n * lpr, 256);
+  if (flags & 2) return (flags & 4) ? (tiles + 7) / 8 * 8 : op.persistent_blocks;
+  return tiles;
 }
 
 template <typename F> bool dispatch_k(int k, F &&f) {
@@ -981,6 +985,10 @@ bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *
 // Times every product schedule (lanes per row x stream-load kind x grid
 // mapping) on the cached matrix and keeps the fastest, separately for the
 // FP64 kernel and the FP32 block-of-8 kernel.
+// plain, non-temporal, die-aware persistent (with / without non-temporal),
+// die-aware over the full grid (with / without non-temporal)
+constexpr int kScheduleFlags[] = {0, 1, 2, 3, 6, 7};
+
 int operator_autotune(OperatorCore &op) {
   Csr const &c = op.csr;
   int64_t const n = c.rows;
@@ -1009,7 +1017,7 @@ int operator_autotune(OperatorCore &op) {
       (void)hipMemsetAsync(x.ptr, 0, sizeof(double) * n, stream());
       float best = 1e30f;
       int best_l = op.lanes_f64, best_f = op.flags_f64;
-      for (int f = 0; f < 4; ++f)
+      for (int f : kScheduleFlags)
         for (int l : cand) {
           op.lanes_f64 = l;
           op.flags_f64 = f;
@@ -1030,7 +1038,7 @@ int operator_autotune(OperatorCore &op) {
       (void)hipMemsetAsync(x.ptr, 0, sizeof(float) * n * 8, stream());
       float best = 1e30f;
       int best_l = op.lanes_f32, best_f = op.flags_f32;
-      for (int f = 0; f < 4; ++f)
+      for (int f : kScheduleFlags)
         for (int l : cand) {
           op.lanes_f32 = l;
           op.flags_f32 = f;
