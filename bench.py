@@ -138,6 +138,8 @@ def sectors_bench(args):
     traffic, traffic_src = profiled_traffic(f"chain{n}_sectors", sector_kernel)
     value = 8.0 * nnz * args.steps / t_steps
     del dx, dy
+    sched4 = batch.autotune(4)  # the eigensolver's products carry 4 complex columns
+    log(f"autotune (4 columns): {sched4}")
     r = batch.eigensolve(nev=4, block_size=4, max_blocks=args.sector_blocks, keep_blocks=args.sector_keep,
                          max_cycles=args.sector_cycles, tol=1e-7, switch_tol=args.sector_switch_tol, seed=1)
     log(f"eigensolve: {r['cycles']} cycles ({r['fp64_cycles']} FP64), {r['products']} products, "
@@ -176,7 +178,7 @@ def sectors_bench(args):
                      "kernel": sector_kernel, "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch},
         "setup_seconds": t_setup,
         "eigensolve": {"method": "complex thick-restart block Lanczos, FP32 then FP64",
-                       "block_complex_columns": 4, "max_blocks": args.sector_blocks,
+                       "block_complex_columns": 4, "schedule": sched4, "max_blocks": args.sector_blocks,
                        "keep_blocks": args.sector_keep, "switch_tol": args.sector_switch_tol,
                        "cycles": r["cycles"], "fp64_cycles": r["fp64_cycles"], "products": r["products"],
                        "product_seconds": r["product_seconds"], "seconds": r["total_seconds"],

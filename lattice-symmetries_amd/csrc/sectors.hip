@@ -63,6 +63,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kDies = 8; // accelerator complex dies; workgroups are dealt round-robin
+constexpr int kMaxGrid = 1 << 26;
 
 // ---------------------------------------------------------------------------
 // The product
@@ -191,7 +192,7 @@ bool launch_product(ls_amd_sector_batch const &b, int form, int k, void const *x
 }
 
 bool check_schedule(int lanes, int blocks, int xcd) {
-  if (blocks < kDies || blocks > (1 << 20)) return fail("the product grid must hold 8 .. 2^20 workgroups");
+  if (blocks < kDies || blocks > kMaxGrid) return fail("the product grid must hold 8 .. 2^26 workgroups");
   if (xcd && blocks % kDies != 0) return fail("a die-aware grid must be a multiple of 8 workgroups");
   if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
     return fail("lanes per row must be 1, 2, 4, 8 or 16");
@@ -361,9 +362,12 @@ bool sector_autotune(ls_amd_sector_batch &b, int k) {
   float best = 1e30f;
   int bl = b.lanes, bb = b.blocks, bx = b.xcd_aware;
   for (int lanes : {1, 2, 4, 8, 16})
-    for (int per_cu : {2, 4, 8, 16})
+    for (int per_cu : {2, 4, 8, 16, 0})
       for (int xcd : {0, 1}) {
-        int const blocks = std::max(kDies, (cu * per_cu / kDies) * kDies);
+        // per_cu 0: one workgroup per row tile (rounded up to a multiple of 8)
+        int64_t const tiles = (n + kThreads / lanes - 1) / (kThreads / lanes);
+        int const blocks = per_cu > 0 ? std::max(kDies, (cu * per_cu / kDies) * kDies)
+                                      : static_cast<int>(std::min<int64_t>((tiles + 7) / 8 * 8, kMaxGrid));
         if (!launch_product(b, 1, k, x.ptr, y.ptr, lanes, blocks, xcd)) return false;
         LSAMD_HIP(hipEventRecord(e0, stream()));
         for (int r = 0; r < 3; ++r)

@@ -53,7 +53,7 @@ def test_sector_product_matches_oracle(engine):
     X = rng.standard_normal((batch.number_states, 8)) + 1j * rng.standard_normal((batch.number_states, 8))
     want = batch.matvec(X, 64)
     for lanes in (1, 2, 4, 8, 16):
-        for blocks, die in ((8, 1), (8, 0), (1024, 1), (1000, 0)):
+        for blocks, die in ((8, 1), (8, 0), (64, 1), (64, 0), (1024, 1), (1000, 0)):
             batch.set_schedule(lanes, blocks, die)
             assert np.abs(batch.matvec(X, 64) - want).max() <= RTOL64 * np.abs(want).max()
     sched = batch.autotune(8)
