@@ -421,15 +421,18 @@ int64_t ls_amd_sector_batch_nnz(ls_amd_sector_batch const *batch);
 /* offsets[count + 1] (host): first state of every sector. */
 int ls_amd_sector_batch_offsets(ls_amd_sector_batch const *batch,
                                 int64_t *offsets);
-/* Product schedule: lanes per row (1, 2, 4, 8, 16), workgroups in the
-   persistent grid, and whether each die's workgroups own one contiguous
-   range of rows (die_aware = 1, grid a multiple of 8). */
+/* Product schedule: lanes per row (1, 2, 4, 8, 16), workgroups in the grid
+   (8 .. 2^26; workgroups loop over row tiles, so a grid of at least one
+   workgroup per tile of 256 / lanes rows gives each workgroup one tile), and
+   whether each die's workgroups own one contiguous range of rows
+   (die_aware = 1, grid a multiple of 8).  One schedule serves every k. */
 int ls_amd_sector_batch_set_schedule(ls_amd_sector_batch *batch, int lanes,
                                      int blocks, int die_aware);
 int ls_amd_sector_batch_get_schedule(ls_amd_sector_batch const *batch,
                                      int *lanes, int *blocks, int *die_aware);
-/* Times every schedule for the FP32 product with k columns and keeps the
-   fastest. */
+/* Times every schedule (lanes x grid of 2, 4, 8, 16 workgroups per compute
+   unit or one per tile x die-aware) for the FP32 product with k columns and
+   keeps the fastest. */
 int ls_amd_sector_batch_autotune(ls_amd_sector_batch *batch, int k);
 /* y = M x on every sector, k in {1, 2, 4, 8} complex columns. */
 int ls_amd_sector_batch_matvec_f32(ls_amd_sector_batch *batch, int k,
