@@ -186,6 +186,29 @@ __global__ __launch_bounds__(kThreads) void normal_kernel(uint64_t seed, T *__re
   x[i] = static_cast<T>(sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2));
 }
 
+// Seeded N(0, 1) entries (as normal_kernel, indexed by the element of the
+// block) in the columns c >= keep[s] of the rows of segment s; a row holds
+// `width` reals, column c of a field entry being real column c (real blocks)
+// or c and field_cols + c (complex blocks stored [re | im]).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void refill_kernel(uint64_t seed, T *__restrict__ x, int width,
+                                                          int field_cols, int64_t const *__restrict__ chunk_rows,
+                                                          int const *__restrict__ chunk_seg,
+                                                          int const *__restrict__ keep) {
+  int const c = blockIdx.x;
+  int const k = keep[chunk_seg[c]];
+  if (k >= field_cols) return;
+  int64_t const e0 = chunk_rows[2 * c] * width, e1 = chunk_rows[2 * c + 1] * width;
+  for (int64_t i = e0 + threadIdx.x; i < e1; i += kThreads) {
+    if (static_cast<int>(i % width) % field_cols < k) continue;
+    uint64_t const a = mix64(seed ^ mix64(2 * static_cast<uint64_t>(i)));
+    uint64_t const b = mix64(seed ^ mix64(2 * static_cast<uint64_t>(i) + 1));
+    double const u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+    double const u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+    x[i] = static_cast<T>(sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2));
+  }
+}
+
 template <typename F> bool dispatch_block(int b, F &&f) {
   switch (b) {
   case 1: return f.template operator()<1>();
@@ -350,6 +373,7 @@ struct BlockAlgebra {
   Segments seg;
   int B = 8;
   DeviceArray<double> partial, d_out, d_coef;
+  DeviceArray<int> d_flags;
   std::vector<double> h_coef;
   int max_nblk = 0;
   bool init(std::vector<int64_t> const &offsets, std::vector<int> const &listed, int b, int nblk_max) {
@@ -381,6 +405,17 @@ struct BlockAlgebra {
     if (seg.chunks > 0)
       LSAMD_HIP(hipMemcpyAsync(out_host.data(), d_out.ptr, sizeof(double) * out_host.size(), hipMemcpyDeviceToHost,
                                stream()));
+    LSAMD_HIP(hipStreamSynchronize(stream()));
+    return true;
+  }
+  // Fresh seeded random entries in the field columns c >= keep[s] of every
+  // segment s (field_cols columns per row).
+  template <typename T> bool refill(T *X, int field_cols, std::vector<int> const &keep, uint64_t seed) {
+    if (seg.chunks == 0) return true;
+    if (!d_flags.upload(keep.data(), keep.size())) return false;
+    refill_kernel<T><<<seg.chunks, kThreads, 0, stream()>>>(seed, X, B, field_cols, seg.d_chunk_rows.ptr,
+                                                            seg.d_chunk_seg.ptr, d_flags.ptr);
+    LSAMD_HIP(hipGetLastError());
     LSAMD_HIP(hipStreamSynchronize(stream()));
     return true;
   }
@@ -506,32 +541,117 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
     for (size_t k = 0; k < static_cast<size_t>(ns) * nb; ++k)
       field_to_coef(M.data() + k * Vf, Bf, coef.data() + k * Vw);
   };
-  // CholQR2 of block X into out (may be X); Rf receives the factor R per segment.
-  auto cholqr = [&]<typename T>(T *X, T *out) -> bool {
-    std::vector<F> Racc(static_cast<size_t>(ns) * Vf, F(0.0)), Minv(static_cast<size_t>(ns) * Vf);
-    for (int s = 0; s < ns; ++s)
-      for (int i = 0; i < Bf; ++i) Racc[static_cast<size_t>(s) * Vf + i * Bf + i] = 1.0;
-    T *src = X;
-    for (int pass = 0; pass < 2; ++pass) {
-      if (!alg.gram(src, blk, 1, src, G)) return false;
-      gram_to_field(G, static_cast<size_t>(ns), Bf, Gf);
-      for (int s = 0; s < ns; ++s) {
-        std::vector<F> Gs(Gf.begin() + static_cast<int64_t>(s) * Vf, Gf.begin() + static_cast<int64_t>(s + 1) * Vf);
-        std::vector<F> R, Rinv;
-        if (!cholesky_upper(Gs, Bf, R, Rinv))
-          return fail("block Lanczos breakdown in segment " + std::to_string(listed[s]) +
-                      " (the Krylov space is exhausted; lower max_blocks or the block size)");
-        std::copy(Rinv.begin(), Rinv.end(), Minv.begin() + static_cast<int64_t>(s) * Vf);
-        std::vector<F> Ra(Racc.begin() + static_cast<int64_t>(s) * Vf, Racc.begin() + static_cast<int64_t>(s + 1) * Vf);
-        std::vector<F> const Rn = matmul(R, Ra, Bf);
-        std::copy(Rn.begin(), Rn.end(), Racc.begin() + static_cast<int64_t>(s) * Vf);
+  // CholQR2 of block X into out (may be X); Rf receives the factor R per
+  // segment (X = out R).  A segment whose Gram matrix is singular has run out
+  // of Krylov directions (A applied to the last block falls inside the
+  // stored blocks, in some or all directions).  Its block keeps the
+  // directions whose Gram eigenvalues stand above the storage precision
+  // (normalised), gets fresh random columns in place of the others,
+  // orthogonalised twice against the nprev stored blocks at Vbase, and goes
+  // through CholQR2 again; R is then formed explicitly as out^H X, so the
+  // Krylov relation and the residual bounds stay exact.  The projected matrix
+  // is filled from explicit projections and needs no change.
+  uint64_t refills = 0;
+  DeviceArray<double> Xorig; // X before a refill (reals of the block's type)
+  auto cholqr = [&]<typename T>(T *X, T *out, T const *Vbase, int nprev) -> bool {
+    constexpr double kKeep = std::is_same_v<T, float> ? 1e-10 : 1e-24; // relative Gram eigenvalue kept
+    std::vector<int> refilled(ns, 0);
+    bool saved = false;
+    std::vector<double> Cr;
+    std::vector<F> Cfr;
+    for (int attempt = 0;; ++attempt) {
+      std::vector<F> Racc(static_cast<size_t>(ns) * Vf, F(0.0)), Minv(static_cast<size_t>(ns) * Vf);
+      for (int s = 0; s < ns; ++s)
+        for (int i = 0; i < Bf; ++i) Racc[static_cast<size_t>(s) * Vf + i * Bf + i] = 1.0;
+      std::vector<int> singular(ns, 0);
+      int first = -1;
+      T *src = X;
+      for (int pass = 0; pass < 2; ++pass) {
+        if (!alg.gram(src, blk, 1, src, G)) return false;
+        gram_to_field(G, static_cast<size_t>(ns), Bf, Gf);
+        for (int s = 0; s < ns; ++s) {
+          std::vector<F> Gs(Gf.begin() + static_cast<int64_t>(s) * Vf, Gf.begin() + static_cast<int64_t>(s + 1) * Vf);
+          std::vector<F> R, Rinv;
+          if (!cholesky_upper(Gs, Bf, R, Rinv)) {
+            // only the first pass sees X itself; the second sees an
+            // orthonormalised block and cannot be singular unless X was
+            if (pass == 1)
+              return fail("block Lanczos breakdown in segment " + std::to_string(listed[s]) +
+                          " (the orthonormalised block is singular)");
+            singular[s] = 1;
+            if (first < 0) first = s;
+            continue;
+          }
+          std::copy(Rinv.begin(), Rinv.end(), Minv.begin() + static_cast<int64_t>(s) * Vf);
+          std::vector<F> Ra(Racc.begin() + static_cast<int64_t>(s) * Vf,
+                            Racc.begin() + static_cast<int64_t>(s + 1) * Vf);
+          std::vector<F> const Rn = matmul(R, Ra, Bf);
+          std::copy(Rn.begin(), Rn.end(), Racc.begin() + static_cast<int64_t>(s) * Vf);
+        }
+        if (first >= 0) break; // X is untouched: repair below and start over
+        set_coef(Minv, 1);
+        if (!alg.combine(out, 0.0, src, blk, 1, coef)) return false;
+        src = out;
       }
-      set_coef(Minv, 1);
-      if (!alg.combine(out, 0.0, src, blk, 1, coef)) return false;
-      src = out;
+      if (first < 0) {
+        Rf = Racc;
+        if (saved) { // R = out^H X for the repaired segments
+          if (!alg.gram(out, blk, 1, reinterpret_cast<T const *>(Xorig.ptr), Cr)) return false;
+          gram_to_field(Cr, static_cast<size_t>(ns), Bf, Cfr);
+          for (int s = 0; s < ns; ++s)
+            if (refilled[s])
+              std::copy(Cfr.begin() + static_cast<int64_t>(s) * Vf, Cfr.begin() + static_cast<int64_t>(s + 1) * Vf,
+                        Rf.begin() + static_cast<int64_t>(s) * Vf);
+        }
+        return true;
+      }
+      if (attempt == 3)
+        return fail("block Lanczos breakdown in segment " + std::to_string(listed[first]) +
+                    " (no independent directions left; lower max_blocks or the block size)");
+      if (!saved) {
+        if (!Xorig.ptr && !Xorig.allocate(static_cast<size_t>(blk))) return false;
+        LSAMD_HIP(hipMemcpyAsync(Xorig.ptr, X, sizeof(T) * blk, hipMemcpyDeviceToDevice, stream()));
+        saved = true;
+      }
+      // keep the independent directions: X <- X M with M = U_K Lambda_K^-1/2
+      // (columns of the kept Gram eigenvectors, largest first, then zeros)
+      std::vector<F> M(static_cast<size_t>(ns) * Vf, F(0.0));
+      std::vector<int> keep(ns, Bf);
+      for (int s = 0; s < ns; ++s) {
+        F *Ms = M.data() + static_cast<size_t>(s) * Vf;
+        if (!singular[s]) {
+          for (int i = 0; i < Bf; ++i) Ms[i * Bf + i] = 1.0;
+          continue;
+        }
+        std::vector<F> Gs(Gf.begin() + static_cast<int64_t>(s) * Vf, Gf.begin() + static_cast<int64_t>(s + 1) * Vf);
+        std::vector<double> lam;
+        std::vector<F> U;
+        if (!field_eigh(Gs, Bf, lam, U)) return fail("block Lanczos: Gram eigen-decomposition failed");
+        double const top = lam[Bf - 1];
+        int k = 0;
+        for (int c = Bf - 1; c >= 0 && top > 0.0 && lam[c] > kKeep * top; --c, ++k)
+          for (int r = 0; r < Bf; ++r) Ms[r * Bf + k] = U[static_cast<size_t>(r) * Bf + c] / std::sqrt(lam[c]);
+        keep[s] = k;
+      }
+      set_coef(M, 1);
+      if (!alg.combine(X, 0.0, X, blk, 1, coef)) return false;
+      // fresh random columns in place of the others, orthogonalised twice
+      // against the stored blocks (zero coefficients keep other segments)
+      ++refills;
+      if (!alg.refill(X, Bf, keep, o.seed ^ (0x5bd1e995ull * refills))) return false;
+      for (int pass = 0; pass < 2 && nprev > 0; ++pass) {
+        if (!alg.gram(Vbase, blk, nprev, X, Cr)) return false;
+        gram_to_field(Cr, static_cast<size_t>(ns) * nprev, Bf, Cfr);
+        for (int s = 0; s < ns; ++s)
+          for (size_t t = 0; t < static_cast<size_t>(nprev) * Vf; ++t) {
+            F &c = Cfr[static_cast<size_t>(s) * nprev * Vf + t];
+            c = singular[s] ? -c : F(0.0);
+          }
+        set_coef(Cfr, nprev);
+        if (!alg.combine(X, 1.0, Vbase, blk, nprev, coef)) return false;
+      }
+      for (int s = 0; s < ns; ++s) refilled[s] |= singular[s];
     }
-    Rf = Racc;
-    return true;
   };
 
   int const dimT = m * Bf;
@@ -601,9 +721,9 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
           }
         }
         if (j + 1 < m) {
-          if (!cholqr(Wt, Vj(j + 1))) return -1;
+          if (!cholqr(Wt, Vj(j + 1), Vbase, nb)) return -1;
         } else {
-          if (!cholqr(Wt, Wt)) return -1; // residual block; Rf holds its factor
+          if (!cholqr(Wt, Wt, Vbase, nb)) return -1; // residual block; Rf holds its factor
         }
       }
       // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
@@ -666,7 +786,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
     float *V32 = reinterpret_cast<float *>(Vb.ptr);
     normal_kernel<float><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, V32, blk);
     LSAMD_HIP(hipGetLastError());
-    if (!cholqr(V32, V32)) return false;
+    if (!cholqr(V32, V32, V32, 0)) return false;
     status = run_cycles(V32, reinterpret_cast<float *>(W.ptr), reinterpret_cast<float *>(Y.ptr));
     if (status < 0) return false;
     if (status == 1) {
@@ -677,12 +797,12 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
       LSAMD_HIP(hipMemcpyAsync(Vb.ptr, Y.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
       for (int s = 0; s < ns; ++s) std::fill(Tp[s].begin(), Tp[s].end(), F(0.0));
       kept = 0;
-      if (!cholqr(Vb.ptr, Vb.ptr)) return false;
+      if (!cholqr(Vb.ptr, Vb.ptr, Vb.ptr, 0)) return false;
     }
   } else {
     normal_kernel<double><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, Vb.ptr, blk);
     LSAMD_HIP(hipGetLastError());
-    if (!cholqr(Vb.ptr, Vb.ptr)) return false;
+    if (!cholqr(Vb.ptr, Vb.ptr, Vb.ptr, 0)) return false;
     status = 1;
   }
   if (status == 1 && run_cycles(Vb.ptr, W.ptr, Y.ptr) < 0) return false;

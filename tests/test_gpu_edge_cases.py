@@ -165,3 +165,30 @@ def test_one_state_and_empty_bases(engine):
     assert op.matvec(np.zeros(0), "csr").size == 0
     with pytest.raises(lsamd.EngineError, match="empty basis"):
         op.lanczos("fp64", max_steps=10)
+
+
+def test_block_solvers_survive_an_exhausted_krylov_space(engine):
+    """sigma^z sigma^z ring: a diagonal operator with few distinct values, so
+    the block Krylov space of any start block is exhausted after a few steps
+    (A V_j falls inside the stored blocks exactly).  The solvers refill such
+    a block with fresh random directions and still return the lowest
+    eigenvalues: -16 twice (the two Neel states) on the 16-site ring."""
+    n, h = 16, 8
+    expr = zz_chain(n)
+    b = lsamd.DeviceBasis(n, h, None, None)
+    b.build()
+    op = lsamd.DeviceOperator(b, expr)
+    op.build_csr(96)
+    for mode in ("fp64", "fp32"):
+        say("exhausted krylov", mode)
+        r = op.block_eigensolve(mode, block_size=8, nev=2, max_blocks=12, keep_blocks=2, max_cycles=10, tol=1e-8)
+        assert r["converged"], r
+        assert np.abs(r["eigenvalues"] - np.array([-16.0, -16.0])).max() < 1e-10
+    sl = [(0, None), (3, None), (8, None)]
+    bases, ops = LS.chain_sector_operators(n, h, expr, sl)
+    ref = H.chain_sector_matrices(n, h, expr, sl, LS.DEFAULT_NORM_TOLERANCE)
+    batch = lsamd.DeviceSectorBatch(ops)
+    r = batch.eigensolve(nev=2, block_size=4, max_blocks=12, keep_blocks=3, max_cycles=10, tol=1e-8)
+    assert r["converged"]
+    for s, (_, M) in enumerate(ref):
+        assert np.abs(r["eigenvalues"][s] - np.linalg.eigvalsh(M)[:2]).max() < 1e-10, (sl[s], r["eigenvalues"][s])
