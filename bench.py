@@ -220,17 +220,19 @@ def gather_ceiling(args):
     out = []
     for packed in (1, 0):
         for window in (1 << 16, 1 << 18, 1 << 20, 1 << 22, rows):
-            for lanes in (8, 16, 32):
-                ms = C.c_double()
-                rc = lib.ls_amd_gather_probe(rows, per_row, window, lanes, packed, 5, C.byref(ms))
-                if rc != 0:
-                    raise SystemExit(lsamd._abi.last_error())
-                nnz = rows * per_row
-                out.append({"packed": packed, "window_columns": window, "window_MB": window * 8 / 1e6,
-                             "lanes": lanes, "ms": ms.value, "gathers_per_s": nnz / (ms.value * 1e-3),
-                             "algorithmic_GBs": (20.0 * nnz + 32.0 * rows) / (ms.value * 1e-3) / 1e9})
-                log(f"packed {packed} window {window:9d} lanes {lanes:2d}: {ms.value:.3f} ms, "
-                    f"{nnz / (ms.value * 1e-3) / 1e9:.1f} G gathers/s")
+            for flags in (0, 6):  # plain grid; die-aware mapping, one workgroup per tile
+                for lanes in (8, 16, 32):
+                    ms = C.c_double()
+                    rc = lib.ls_amd_gather_probe(rows, per_row, window, lanes, flags, packed, 5, C.byref(ms))
+                    if rc != 0:
+                        raise SystemExit(lsamd._abi.last_error())
+                    nnz = rows * per_row
+                    out.append({"packed": packed, "window_columns": window, "window_MB": window * 8 / 1e6,
+                                "flags": flags, "lanes": lanes, "ms": ms.value,
+                                "gathers_per_s": nnz / (ms.value * 1e-3),
+                                "algorithmic_GBs": (20.0 * nnz + 32.0 * rows) / (ms.value * 1e-3) / 1e9})
+                    log(f"packed {packed} window {window:9d} flags {flags} lanes {lanes:2d}: {ms.value:.3f} ms, "
+                        f"{nnz / (ms.value * 1e-3) / 1e9:.1f} G gathers/s")
     print(json.dumps({"workload": "gather-ceiling", "rows": rows, "per_row": per_row, "results": out}), flush=True)
 
 

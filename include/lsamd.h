@@ -494,10 +494,11 @@ ls_amd_operator *ls_amd_model_create_operator(ls_amd_model const *model,
 /* Measurement helper (no reference counterpart): times the FP64 CSR product
    kernel on a synthetic matrix of `rows` rows x `per_row` entries whose
    rows gather from windows of `window` columns (window = rows: the whole
-   vector), plain (packed = 0) or value-table format.  ms_out: mean duration
-   of `reps` launches. */
+   vector), plain (packed = 0) or value-table format, with `lanes` per row
+   and the schedule `flags` of ls_amd_operator_set_schedule.  ms_out: mean
+   duration of `reps` launches. */
 int ls_amd_gather_probe(int64_t rows, int per_row, int64_t window, int lanes,
-                        int packed, int reps, double *ms_out);
+                        int flags, int packed, int reps, double *ms_out);
 
 /* Timing of the last launched kernel of the given family, in milliseconds
    (HIP events on the engine's stream). family: 0 csr f64, 1 csr f32 block,

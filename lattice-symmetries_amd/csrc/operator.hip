@@ -800,7 +800,23 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
     double acc = 0.0;
     if (row < n) {
       int64_t const b = rp[row], e = rp[row + 1];
-      for (int64_t k = b + sub; k < e; k += LPR) {
+      int64_t k = b + sub;
+      // four entries per step: their gathers are in flight together; the
+      // sum keeps the entries' order
+      for (; k + 3 * LPR < e; k += 4 * LPR) {
+        uint32_t c0, c1, c2, c3;
+        double v0, v1, v2, v3;
+        csr_entry<PACKED, NT>(cols, vals, tab, k, c0, v0);
+        csr_entry<PACKED, NT>(cols, vals, tab, k + LPR, c1, v1);
+        csr_entry<PACKED, NT>(cols, vals, tab, k + 2 * LPR, c2, v2);
+        csr_entry<PACKED, NT>(cols, vals, tab, k + 3 * LPR, c3, v3);
+        double const x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        acc = fma(v0, x0, acc);
+        acc = fma(v1, x1, acc);
+        acc = fma(v2, x2, acc);
+        acc = fma(v3, x3, acc);
+      }
+      for (; k < e; k += LPR) {
         uint32_t col;
         double v;
         csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
@@ -838,7 +854,27 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const
     for (int q = 0; q < K; ++q) acc[q] = T(0);
     if (row < n) {
       int64_t const b = rp[row], e = rp[row + 1];
-      for (int64_t k = b + sub; k < e; k += LPR) {
+      int64_t k = b + sub;
+      // two entries per step, their gathers in flight together, summed in order
+      for (; k + LPR < e; k += 2 * LPR) {
+        uint32_t c0, c1;
+        T v0, v1;
+        csr_entry<PACKED, NT>(cols, vals, tab, k, c0, v0);
+        csr_entry<PACKED, NT>(cols, vals, tab, k + LPR, c1, v1);
+        T const *x0 = x + static_cast<size_t>(c0) * K;
+        T const *x1 = x + static_cast<size_t>(c1) * K;
+        T a0[K], a1[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          a0[q] = x0[q];
+          a1[q] = x1[q];
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] = fma(v0, a0[q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] = fma(v1, a1[q], acc[q]);
+      }
+      for (; k < e; k += LPR) {
         uint32_t col;
         T v;
         csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
@@ -1095,11 +1131,15 @@ __global__ __launch_bounds__(256) void probe_cols_kernel(int64_t rows, int per_r
 } // namespace
 } // namespace lsamd
 
-extern "C" int ls_amd_gather_probe(int64_t rows, int per_row, int64_t window, int lanes, int packed, int reps,
-                                   double *ms_out) {
+extern "C" int ls_amd_gather_probe(int64_t rows, int per_row, int64_t window, int lanes, int flags, int packed,
+                                   int reps, double *ms_out) {
   using namespace lsamd;
   if (ms_out == nullptr || rows <= 0 || per_row <= 0 || window <= 0 || window > rows || reps <= 0)
     return fail("ls_amd_gather_probe: bad arguments") ? 0 : -1;
+  if (flags < 0 || flags > 7 || flags == 4 || flags == 5)
+    return fail("ls_amd_gather_probe: schedule flags are 0..3, 6 or 7") ? 0 : -1;
+  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0)
+    return fail("ls_amd_gather_probe: lanes per row must be a power of two <= 64") ? 0 : -1;
   if (rows >= (int64_t{1} << 24) && packed) return fail("ls_amd_gather_probe: packed needs rows < 2^24") ? 0 : -1;
   if (rows >= (int64_t{1} << 32) - 1) return fail("ls_amd_gather_probe: too many rows") ? 0 : -1;
   if (!ensure_device()) return -1;
@@ -1128,7 +1168,7 @@ extern "C" int ls_amd_gather_probe(int64_t rows, int per_row, int64_t window, in
   }
   c.has64 = true;
   op.lanes_f64 = lanes;
-  op.flags_f64 = 0;
+  op.flags_f64 = flags;
   if (!operator_csr_f64(op, x.ptr, y.ptr, false)) return -1;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
