@@ -13,6 +13,7 @@ restatement of the reference's CPU algorithm, on a bounded sample).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload chain-sectors   (56 sectors of the 28-site chain)
+       python bench.py --workload configs         (16- and 32-site chains, 4x4 Hubbard)
 """
 import argparse
 import json
@@ -191,6 +192,107 @@ def sectors_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def timed(lib, family, steps, launch):
+    """Mean engine-timer duration (ms) of `launch` over `steps` launches after one warm-up."""
+    launch()
+    lib.ls_amd_synchronize()
+    ms = []
+    for _ in range(steps):
+        launch()
+        ms.append(lib.ls_amd_last_kernel_ms(family))
+    lib.ls_amd_synchronize()
+    return float(np.mean(ms))
+
+
+def configs_bench(args):
+    """The three smaller workloads, one JSON line each: the 16-site chain
+    (k = 0, parity +, inversion +, S.S couplings, 100-step FP64 Lanczos), the
+    32-site chain (k = 0, parity +, inversion +) and the 4x4 Hubbard model
+    (t = 1, U = 4, 7 + 7 fermions, T x C4v A1 with fermionic signs, complex
+    x).  Each reports its basis build, its CSR build and its products against
+    the HBM roofline."""
+    import lsamd
+    from lsamd import models
+    lib = lsamd.load()
+    rng = np.random.default_rng(42)
+
+    def spin_chain(n, expr):
+        gens = [models.chain_translation(n), models.chain_parity(n)]
+        S = lsamd.Symmetries([lsamd.Symmetry(g, 0) for g in gens])
+        return lsamd.DeviceBasis(n, n // 2, 1, S), expr
+
+    def hubbard(L):
+        n = L * L
+        gens = [np.concatenate([g, g + n]).astype(np.int32) for g in models.square_generators(L)]
+        S = lsamd.Symmetries([lsamd.Symmetry(g, 0) for g in gens])
+        return lsamd.DeviceBasis.fermions(n, 7, 7, S), models.hubbard(n, models.square_nn_bonds(L), t=1.0, U=4.0)
+
+    cases = [("chain16", "Heisenberg chain N=16, H = sum S_i.S_i+1, Hamming weight 8, k=0, parity +, inversion + "
+                         "(64 elements)", lambda: spin_chain(16, models.heisenberg_chain(16, 1.0, 0.25))),
+             ("chain32", "Heisenberg chain N=32, J=1 (sigma.sigma), Hamming weight 16, k=0, parity +, inversion + "
+                         "(128 elements)", lambda: spin_chain(32, models.heisenberg_chain(32))),
+             ("hubbard4x4", "Hubbard 4x4, t=1, U=4, 7 up + 7 down, T x C4v A1 with fermionic signs "
+                            "(128 elements), complex x", lambda: hubbard(4))]
+    for name, desc, make in cases:
+        basis, expr = make()
+        lib.ls_amd_synchronize()
+        t0 = time.perf_counter()
+        dim = basis.build()
+        lib.ls_amd_synchronize()
+        t_build = time.perf_counter() - t0
+        op = lsamd.DeviceOperator(basis, expr)
+        t0 = time.perf_counter()
+        nnz_off = op.build_csr(96)
+        t_csr = time.perf_counter() - t0
+        op.autotune()
+        sched = op.schedule
+        nnz = nnz_off + dim
+        line = {"metric": f"{name}: H·x nonzeros/s and basis states/s", "unit": "nonzeros/s", "n_gpus": 1,
+                "steps": args.steps, "higher_is_better": True, "dtype": "f64",
+                "data": "synthetic: seeded N(0,1) x (seed 42)",
+                "config": {"workload": desc, "states": dim, "off_diagonal_nonzeros": nnz_off, "schedule": sched},
+                "basis_build": {"seconds": t_build, "states_per_s": dim / t_build,
+                                "candidates": basis.number_candidates,
+                                "candidates_per_s": basis.number_candidates / t_build},
+                "csr_build_seconds": t_csr}
+        if name == "hubbard4x4":
+            # complex x on the real A1 matrix: the interleaved complex vector
+            # is an [n][2] real block, so the FP64 block product with k = 2
+            # is the complex product (28 B per nonzero: column, value, x)
+            xc = rng.standard_normal(dim) + 1j * rng.standard_normal(dim)
+            dx = lsamd.DeviceArray.from_numpy(xc)
+            dy = lsamd.DeviceArray(xc.nbytes)
+            dz = lsamd.DeviceArray(xc.nbytes)
+            k_ms = timed(lib, 5, args.steps, lambda: op.csr_matvec_f64_block(2, dx, dy))
+            otf_ms = timed(lib, 2, max(2, args.steps // 4), lambda: op.matvec_complex(dx, dz))
+            y, z = dy.to_numpy(np.complex128, (dim,)), dz.to_numpy(np.complex128, (dim,))
+            line["csr_vs_otf_max_rel_diff"] = float(np.abs(y - z).max() / np.abs(z).max())
+            bytes_per_launch = 28.0 * nnz_off + 48.0 * dim
+            kernel = f"csr_block_kernel<2, {sched['lanes_f64']}, double> flags {sched['flags_f64']} (complex x)"
+            line["otf_complex"] = {"kernel_ms": otf_ms, "nonzeros_per_s": nnz / (otf_ms * 1e-3)}
+        else:
+            x = lsamd.DeviceArray.from_numpy(rng.standard_normal(dim))
+            y = lsamd.DeviceArray(8 * dim)
+            k_ms = timed(lib, 0, args.steps, lambda: op.csr_matvec(x, y))
+            otf_ms = timed(lib, 2, max(2, args.steps // 4), lambda: op.matvec_otf(x, y))
+            bytes_per_launch = 20.0 * nnz_off + 32.0 * dim
+            kernel = f"csr_f64_kernel<{sched['lanes_f64']}> flags {sched['flags_f64']}"
+            line["otf"] = {"kernel_ms": otf_ms, "nonzeros_per_s": nnz / (otf_ms * 1e-3)}
+            if name == "chain16":
+                r = op.lanczos("fp64", max_steps=100, k=1, tol=0.0)
+                line["lanczos_100"] = {"E0": r["eigenvalues"][0], "E0_per_site": r["eigenvalues"][0] / 16,
+                                       "steps": r["steps"], "seconds": r["total_seconds"]}
+        achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+        line.update({"value": nnz / (k_ms * 1e-3), "ms_per_step": k_ms,
+                     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                                  "kernel_ms": k_ms, "algorithmic_bytes_per_launch": bytes_per_launch}})
+        log(f"{name}: {dim} states in {t_build:.3f} s, {nnz_off} off-diagonal nonzeros, product {k_ms:.3f} ms "
+            f"({achieved / HBM_PEAK_GBS:.2f} of the roofline), on the fly {otf_ms:.3f} ms")
+        print(json.dumps(line), flush=True)
+        del op, basis
+
+
 def copy_calibration(args):
     """A plain device copy of known size (torch's copy kernel), run under the
     profiler's FETCH_SIZE / WRITE_SIZE counters to scale them to bytes."""
@@ -247,7 +349,8 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--block-cycle", type=int, default=12)
-    ap.add_argument("--workload", choices=["square6", "chain-sectors", "copy-calibration", "gather-ceiling"],
+    ap.add_argument("--workload", choices=["square6", "chain-sectors", "configs", "copy-calibration",
+                                           "gather-ceiling"],
                     default="square6")
     ap.add_argument("--copy-mib", type=int, default=2048)
     ap.add_argument("--chain", type=int, default=28)
@@ -277,6 +380,9 @@ def main():
         return
     if args.workload == "gather-ceiling":
         gather_ceiling(args)
+        return
+    if args.workload == "configs":
+        configs_bench(args)
         return
 
     n, h, S, expr = workload(args.lattice)
