@@ -32,7 +32,7 @@ constexpr int kSpinful = -2;     // hamming: product of two fixed-weight species
 struct EnumParams {
   uint64_t rank0;   // rank of the first candidate (or the state itself)
   int64_t total;    // number of candidates
-  int64_t per_lane; // candidates per lane, multiple of 32
+  int64_t per_lane; // candidates per lane (acceptance bits: ceil(per_lane / 32) words per lane)
   int64_t lanes;
   int hamming; // >= 0 fixed weight, kAllIntegers, kSpinful
   uint64_t const *binom;
@@ -46,6 +46,8 @@ struct EnumParams {
   // accepts a candidate that passes `limit` elements without the norm test
   int limit, survive_only;
 };
+
+__host__ __device__ constexpr int64_t words_per_lane(int64_t per_lane) { return (per_lane + 31) / 32; }
 
 __device__ __forceinline__ uint64_t enum_first(EnumParams const &p, int64_t first);
 
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
   int64_t remaining = 0;
   if (lane < p.lanes && first < p.total) remaining = min(p.per_lane, p.total - first);
   W x = remaining > 0 ? static_cast<W>(enum_first(p, first)) : W{0};
-  uint32_t *my_bits = bits + lane * (p.per_lane / 32);
+  uint32_t *my_bits = bits + lane * words_per_lane(p.per_lane);
   W const flip = static_cast<W>(g.flip);
   int const nch4 = (G / kIsRepresentativeLanes) * kIsRepresentativeLanes;
   int const limit = p.limit;
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(256) void enum_write_kernel(EnumParams p,
   int64_t const n = min(p.per_lane, p.total - first);
   uint64_t x = enum_first(p, first);
   uint64_t o = offsets != nullptr ? offsets[lane] : static_cast<uint64_t>(first);
-  uint32_t const *my_bits = bits != nullptr ? bits + lane * (p.per_lane / 32) : nullptr;
+  uint32_t const *my_bits = bits != nullptr ? bits + lane * words_per_lane(p.per_lane) : nullptr;
   uint32_t word = 0xffffffffu;
   for (int64_t k = 0; k < n; ++k) {
     if (my_bits != nullptr && (k & 31) == 0) word = my_bits[k >> 5];
@@ -426,10 +428,14 @@ namespace {
 // goes to the full test.
 constexpr int kSurvivorElements = 24;
 
-int64_t lanes_for(int64_t total, int64_t &per_lane) {
+// Candidates per lane for about 2^21 lanes.  Ranked candidates come at least
+// 64 to a lane (a lane unranks its first candidate, then steps); listed
+// survivors of the first pass each cost a full scan of the group, so they
+// may go one to a lane.
+int64_t lanes_for(int64_t total, int64_t &per_lane, bool listed = false) {
   int64_t const target_lanes = int64_t{1} << 21;
   per_lane = (total + target_lanes - 1) / target_lanes;
-  per_lane = std::max<int64_t>(64, ((per_lane + 31) / 32) * 32);
+  per_lane = listed ? std::max<int64_t>(1, per_lane) : std::max<int64_t>(64, ((per_lane + 31) / 32) * 32);
   return (total + per_lane - 1) / per_lane;
 }
 } // namespace
@@ -516,7 +522,7 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
     int const nblocks = grid_for(q.lanes, 256);
     DeviceArray<uint32_t> bits, counts;
     DeviceArray<uint64_t> offsets;
-    if (!bits.allocate(static_cast<size_t>(q.lanes * (q.per_lane / 32)))) return false;
+    if (!bits.allocate(static_cast<size_t>(q.lanes * words_per_lane(q.per_lane)))) return false;
     if (!counts.allocate(static_cast<size_t>(q.lanes))) return false;
     if (!offsets.allocate(static_cast<size_t>(q.lanes))) return false;
     bool ok = dispatch_group(g, [&]<typename W, int D, int SI>() -> bool {
@@ -563,7 +569,7 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
     q.total = static_cast<int64_t>(nsurv);
     q.limit = g.size;
     q.survive_only = 0;
-    q.lanes = lanes_for(q.total, q.per_lane);
+    q.lanes = lanes_for(q.total, q.per_lane, true);
     if (nsurv > 0) {
       if (!filter(q, out, total)) return false;
     } else {
