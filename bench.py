@@ -347,6 +347,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lattice", type=int, default=6)
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-reference-abi", action="store_true",
+                    help="skip timing ls_chpl_matrix_vector_product with host vectors")
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--block-cycle", type=int, default=12)
     ap.add_argument("--workload", choices=["square6", "chain-sectors", "configs", "copy-calibration",
@@ -452,6 +454,33 @@ def main():
     block_bytes = 40.0 * nnz_off + dim * (8 + 4 + 64)
     block_gbs = block_bytes / (float(np.mean(b_ms)) * 1e-3) / 1e9
 
+    # ---- the reference-named entry point: host x and y, copied over PCIe on every call ----
+    ref_abi = None
+    if not args.no_reference_abi:
+        sb = lsamd.SpinBasis(n, h, 1, S)
+        sb.build()
+        hop = lsamd.Operator(sb, expr)
+        xh = rng.standard_normal(dim)
+        yh = np.zeros(dim)
+        t0 = time.perf_counter()
+        hop.matvec(xh, out=yh)  # the first call uploads the basis and builds the cached CSR
+        t_first = time.perf_counter() - t0
+        calls = 5
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            hop.matvec(xh, out=yh)
+        t_call = (time.perf_counter() - t0) / calls
+        dxh = lsamd.DeviceArray.from_numpy(xh)
+        op.csr_matvec(dxh, y)
+        yd = y.to_numpy(np.float64, (dim,))
+        ref_abi = {"entry_point": "ls_chpl_matrix_vector_product", "seconds_per_call": t_call,
+                   "nonzeros_per_s_pcie_inclusive": nnz / t_call, "first_call_seconds": t_first,
+                   "max_rel_diff_to_device_product": float(np.abs(yh - yd).max() / np.abs(yd).max()),
+                   "note": "x and y are host arrays, copied to and from the GPU on every call"}
+        log(f"reference entry point: {1e3 * t_call:.1f} ms per call with host x and y "
+            f"({nnz / t_call:.3e} nonzeros/s), first call {t_first:.1f} s")
+        del hop, sb, dxh
+
     # ---- on-the-fly fused FP64 (no CSR) ----
     t0 = time.perf_counter()
     op.matvec_otf(x, y)
@@ -502,6 +531,7 @@ def main():
         "basis_build": {"states": dim, "candidates": candidates, "seconds": t_build,
                         "states_per_s": dim / t_build, "candidates_per_s": candidates / t_build},
         "csr_build_seconds": t_csr,
+        "reference_abi_host_vectors": ref_abi,
         "fp32_block8": {"nonzero_rhs_per_s": block_rate, "speedup_vs_fp64": block_rate / value,
                         "kernel_ms": float(np.mean(b_ms)), "achieved_GBs": block_gbs},
         "otf_fp64": {"seconds": t_otf, "nonzeros_per_s": nnz / t_otf},
