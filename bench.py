@@ -350,7 +350,7 @@ def main():
     ap.add_argument("--no-reference-abi", action="store_true",
                     help="skip timing ls_chpl_matrix_vector_product with host vectors")
     ap.add_argument("--lanczos-steps", type=int, default=200)
-    ap.add_argument("--block-cycle", type=int, default=12)
+    ap.add_argument("--block-cycle", type=int, default=8)
     ap.add_argument("--workload", choices=["square6", "chain-sectors", "configs", "copy-calibration",
                                            "gather-ceiling"],
                     default="square6")
