@@ -222,11 +222,11 @@ bool graph_product(ls_amd_sector_batch &b, int form, int k, void const *x, void 
     bool const ok = launch_product(b, form, k, x, y, b.lanes, b.blocks, b.xcd_aware);
     hipGraph_t gr = nullptr;
     hipError_t const e = hipStreamEndCapture(stream(), &gr);
-    if (!ok) {
+    if (!ok || e != hipSuccess) {
       if (gr != nullptr) (void)hipGraphDestroy(gr);
-      return false;
+      if (!ok) return false;
+      LSAMD_HIP(e);
     }
-    LSAMD_HIP(e);
     hipGraphExec_t ex = nullptr;
     hipError_t const ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
     if (ei != hipSuccess) {

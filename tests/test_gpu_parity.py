@@ -319,3 +319,38 @@ def test_lanczos_precision_forms(engine):
     assert np.allclose(rb["eigenvalues"][1:4], exact[1:4], rtol=1e-5)
     rotf = dop.lanczos("otf", max_steps=300, k=1, tol=1e-13)
     assert abs(rotf["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
+
+
+def test_reference_kernels_are_reentrant(engine):
+    """The reference's Chapel layer calls state_info, is_representative and
+    state_index concurrently from its tasks, each with its own buffers
+    (chapel/src/DistributedMatrixVector.chpl:929-965): eight host threads
+    calling the reference-named entry points at once give the serial results
+    bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+    n, h = 24, 12
+    S = symmetries([models.chain_translation(n)], [5])  # complex characters
+    sb = lsamd.SpinBasis(n, h, 1, S)
+    sb.build()
+    reps = sb.states.copy()
+    rng = np.random.default_rng(24)
+    batches = []
+    for _ in range(32):
+        bits = np.argsort(rng.random((3000, n)), axis=1)[:, :h].astype(np.uint64)
+        states = np.bitwise_or.reduce(np.uint64(1) << bits, axis=1)
+        batches.append(np.concatenate([states, reps[rng.choice(reps.size, 1000)]]))
+
+    def run(xs):
+        r, c, nrm = sb.state_info(xs)
+        f, fn = sb.is_representative(xs)
+        return r, c, nrm, f, fn, sb.index(xs)
+
+    serial = [run(b) for b in batches]
+    assert any((s[5] >= 0).any() for s in serial) and any((s[5] < 0).any() for s in serial)
+    for rnd in range(2):
+        say("concurrent round", rnd)
+        with ThreadPoolExecutor(max_workers=8) as pool:
+            got = list(pool.map(run, batches))
+        for g, s in zip(got, serial):
+            for a, b in zip(g, s):
+                assert np.array_equal(a, b, equal_nan=True)
