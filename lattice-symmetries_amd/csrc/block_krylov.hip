@@ -11,8 +11,8 @@
 // Blocks are [rows][B] row-major FP64.  Per block step: one product (FP32
 // or FP64, supplied by the caller), full re-orthogonalisation against every
 // stored block by segmented Gram reductions (fixed-order two-pass sums) and
-// block updates (two passes on FP64 blocks; on FP32 blocks the second only
-// after a large norm drop), CholQR2 per segment, and the projected matrix
+// block updates (a second pass only after a large norm drop), CholQR2 per
+// segment, and the projected matrix
 // filled from the re-orthogonalisation coefficients.  At the end of a
 // cycle each segment's projected matrix is diagonalised on a host thread and
 // the lowest `keep` Ritz blocks plus the residual block restart the next
@@ -48,13 +48,13 @@ double seconds_since(Clock::time_point t) { return std::chrono::duration<double>
 
 constexpr int kChunkRows = 2048;
 
-// While the blocks are FP32 a block step re-orthogonalises once, and a second
-// time only when the first pass removed all but kFp32Drop of some column's
-// squared norm (then one pass leaves mostly rounding error, as in an
-// exhausted Krylov space).  The FP32 phase only has to bring the Ritz block
-// near the FP32 residual floor; the FP64 phase restarts explicitly from that
-// block and always re-orthogonalises twice.
-constexpr double kFp32Drop = 1e-6;
+// A block step re-orthogonalises once, and a second time only when the first
+// pass removed all but kReorthDrop of some column's squared norm: then one
+// pass leaves mostly rounding error (an exhausted Krylov space).  Otherwise
+// the orthogonality lost in one pass stays near the storage precision times
+// 10^3, and the reported eigenvalues and residuals are explicit FP64 Rayleigh
+// quotients and residual norms in any case.
+constexpr double kReorthDrop = 1e-6;
 
 // partial[(c * nblk + i) * B*B + p*B + q] = sum over rows r of chunk c of
 // A_i[r][p] * W[r][q], with A_i = A + i * a_stride.  Blocks are stored as TA /
@@ -701,26 +701,24 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         ++blocks_done;
         int const nb = j + 1;
         Csum.assign(static_cast<size_t>(ns) * nb * Vf, F(0.0));
-        // FP64 blocks: two passes.  FP32 blocks: the second pass only when
-        // the first removed nearly all of some column (norm drop below
-        // kFp32Drop, by Pythagoras from ||W||^2 and V^H W), where one pass
-        // leaves mostly rounding error (an exhausted Krylov space).
-        bool second = !fp32;
-        if (fp32) {
-          if (!alg.gram(Wt, blk, 1, Wt, Gw)) return -1;
-          gram_to_field(Gw, static_cast<size_t>(ns), Bf, Gwf);
-        }
+        // The second pass only when the first removed nearly all of some
+        // column (squared norm left below kReorthDrop of ||W||^2, by
+        // Pythagoras from ||W||^2 and V^H W), where one pass leaves mostly
+        // rounding error (an exhausted Krylov space, a converged direction).
+        bool second = false;
+        if (!alg.gram(Wt, blk, 1, Wt, Gw)) return -1;
+        gram_to_field(Gw, static_cast<size_t>(ns), Bf, Gwf);
         for (int pass = 0; pass < (second ? 2 : 1); ++pass) {
           if (!alg.gram(Vbase, blk, nb, Wt, C)) return -1;
           gram_to_field(C, static_cast<size_t>(ns) * nb, Bf, Cf);
-          if (fp32 && pass == 0) {
+          if (pass == 0) {
             for (int s = 0; s < ns && !second; ++s)
               for (int q = 0; q < Bf && !second; ++q) {
                 double const before = Field<F>::re(Gwf[static_cast<size_t>(s) * Vf + q * Bf + q]);
                 double proj = 0.0;
                 for (int i = 0; i < nb; ++i)
                   for (int a = 0; a < Bf; ++a) proj += std::norm(Cf[(static_cast<size_t>(s) * nb + i) * Vf + a * Bf + q]);
-                if (before - proj < kFp32Drop * before) second = true;
+                if (before - proj < kReorthDrop * before) second = true;
               }
           }
           for (size_t t = 0; t < Cf.size(); ++t) {
