@@ -123,3 +123,27 @@ def test_square6_product_properties(square6):
     assert np.abs(W64[:, 0] - y1).max() <= 1e-12 * scale
     W32 = op.matvec_block(X, 32)
     assert np.abs(W32 - W64).max() <= 2e-6 * np.abs(W64).max()
+    del X, W64, W32
+
+
+def test_square6_through_the_reference_entry_points(square6):
+    """The drop-in path at full size: the reference-named basis build
+    (ls_hs_build_representatives -> ls_chpl_enumerate_representatives) and
+    product (ls_chpl_matrix_vector_product, host vectors) give the device
+    basis and the device product."""
+    S, b, count, og = square6
+    expr = models.j1j2_square(6, 1.0, 0.5)
+    say("square6: reference entry points")
+    sb = lsamd.SpinBasis(N_SITES, WEIGHT, 1, S)
+    sb.build()
+    assert sb.number_states == DIM and np.array_equal(sb.states, b.states)
+    hop = lsamd.Operator(sb, expr)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(DIM)
+    y = hop.matvec(x)
+    op = lsamd.DeviceOperator(b, expr)
+    op.build_csr(64)
+    want = op.matvec(x, "csr")
+    assert np.abs(y - want).max() <= 1e-12 * np.abs(want).max()
+    # a second call reuses the cached operator
+    assert np.array_equal(hop.matvec(x), y)
