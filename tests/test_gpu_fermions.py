@@ -87,6 +87,11 @@ def test_hubbard_4x4_a1_full_size(engine):
     assert count == 1022734  # fermionic Burnside count (tests/test_fermions.py)
     reps = db.states
     assert np.all(reps[1:] > reps[:-1])
+    # every entry passes the device's is_representative (a kernel separate
+    # from the enumeration): with the count and the strict order, the list is
+    # exactly the sorted list of all representatives
+    f_all, n_all = db.is_representative(reps)
+    assert np.all(f_all == 1) and np.all(n_all > 0)
     rng = np.random.default_rng(5)
     sample = reps[rng.choice(reps.size, 5000, replace=False)]
     f, _ = og.is_representative(sample)

@@ -33,6 +33,12 @@ def test_chain32_basis(chain32):
     assert count == 4707969  # Burnside (tests/test_dimensions.py)
     reps = b.states
     assert np.all(reps[1:] > reps[:-1])
+    # every entry is its own orbit minimum with a nonzero norm (the device's
+    # state_info, a kernel separate from the enumeration): with the Burnside
+    # count and the strict order, the list is exactly the sorted list of all
+    # representatives
+    r_all, _, n_all = b.state_info(reps)
+    assert np.array_equal(r_all, reps) and np.all(n_all > 0)
     rng = np.random.default_rng(3)
     sample = reps[rng.choice(reps.size, size=20000, replace=False)]
     og = O.Group(n, S.elements(), S.characters.real, S.characters.imag, 1)
