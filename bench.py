@@ -192,6 +192,14 @@ def sectors_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def sched_suffix(sched, packed):
+    """Template flags of an FP64 product kernel: non-temporal loads, die-aware
+    mapping, value table (the packed format)."""
+    f = sched["flags_f64"]
+    tf = lambda b: "true" if b else "false"  # noqa: E731
+    return f"{tf(f & 1)}, {tf(f & 2)}, {tf(packed)}"
+
+
 def timed(lib, family, steps, launch):
     """Mean engine-timer duration (ms) of `launch` over `steps` launches after one warm-up."""
     launch()
@@ -246,6 +254,7 @@ def configs_bench(args):
         t_csr = time.perf_counter() - t0
         op.autotune()
         sched = op.schedule
+        packed = op.csr_info["packed"]
         nnz = nnz_off + dim
         line = {"metric": f"{name}: H·x nonzeros/s and basis states/s", "unit": "nonzeros/s", "n_gpus": 1,
                 "steps": args.steps, "higher_is_better": True, "dtype": "f64",
@@ -268,7 +277,7 @@ def configs_bench(args):
             y, z = dy.to_numpy(np.complex128, (dim,)), dz.to_numpy(np.complex128, (dim,))
             line["csr_vs_otf_max_rel_diff"] = float(np.abs(y - z).max() / np.abs(z).max())
             bytes_per_launch = 28.0 * nnz_off + 48.0 * dim
-            kernel = f"csr_block_kernel<2, {sched['lanes_f64']}, double> flags {sched['flags_f64']} (complex x)"
+            kernel = f"csr_block_kernel<2, {sched['lanes_f64']}, double, double, {sched_suffix(sched, packed)}> (complex x)"
             line["otf_complex"] = {"kernel_ms": otf_ms, "nonzeros_per_s": nnz / (otf_ms * 1e-3)}
         else:
             x = lsamd.DeviceArray.from_numpy(rng.standard_normal(dim))
@@ -276,16 +285,20 @@ def configs_bench(args):
             k_ms = timed(lib, 0, args.steps, lambda: op.csr_matvec(x, y))
             otf_ms = timed(lib, 2, max(2, args.steps // 4), lambda: op.matvec_otf(x, y))
             bytes_per_launch = 20.0 * nnz_off + 32.0 * dim
-            kernel = f"csr_f64_kernel<{sched['lanes_f64']}> flags {sched['flags_f64']}"
+            kernel = f"csr_f64_kernel<{sched['lanes_f64']}, {sched_suffix(sched, packed)}>"
             line["otf"] = {"kernel_ms": otf_ms, "nonzeros_per_s": nnz / (otf_ms * 1e-3)}
             if name == "chain16":
                 r = op.lanczos("fp64", max_steps=100, k=1, tol=0.0)
                 line["lanczos_100"] = {"E0": r["eigenvalues"][0], "E0_per_site": r["eigenvalues"][0] / 16,
                                        "steps": r["steps"], "seconds": r["total_seconds"]}
         achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+        if sched["flags_f64"] & 6 == 6:
+            kernel += " full grid"
+        traffic, traffic_src = profiled_traffic(name, kernel)
         line.update({"value": nnz / (k_ms * 1e-3), "ms_per_step": k_ms,
                      "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                                  **traffic_rate(traffic, k_ms), "kernel": kernel,
                                   "kernel_ms": k_ms, "algorithmic_bytes_per_launch": bytes_per_launch}})
         log(f"{name}: {dim} states in {t_build:.3f} s, {nnz_off} off-diagonal nonzeros, product {k_ms:.3f} ms "
             f"({achieved / HBM_PEAK_GBS:.2f} of the roofline), on the fly {otf_ms:.3f} ms")
