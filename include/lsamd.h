@@ -176,6 +176,17 @@ void ls_chpl_operator_apply_off_diag(ls_hs_operator *matrixPtr, int64_t count,
    assigns y; without diagonal terms y is accumulated into (:1054-1057). */
 void ls_chpl_matrix_vector_product(ls_hs_operator *matrixPtr, int numVectors,
                                    double const *xPtr, double *yPtr);
+/* The product above keeps a device copy of each operator it serves (the
+   CSR cache).  An entry is rebuilt when the basis, its representatives,
+   the engine's group / index data behind it or the terms change; it is
+   dropped when that group or index data is destroyed, and the entries
+   together hold at most LSAMD_CACHE_BYTES (default: half the device
+   memory) of device memory, least recently used first out.  No counterpart
+   in the reference, which recomputes every product from scratch:
+   ls_amd_forget_operator drops the entry of an operator its caller is
+   about to destroy; ls_amd_operator_cache_entries counts the entries. */
+void ls_amd_forget_operator(ls_hs_operator const *op);
+int64_t ls_amd_operator_cache_entries(void);
 
 /* ======================================================================== */
 /* Part 2: additive entry points (device-resident data)                      */
@@ -271,10 +282,12 @@ ls_amd_operator *ls_amd_operator_create(ls_amd_basis const *b,
                                         ls_hs_nonbranching_terms const *off_diag);
 void ls_amd_operator_destroy(ls_amd_operator *op);
 
-/* On-the-fly product in pull form (row i gathers from the representatives
-   of H|alpha_i>), FP64, real x / y on the device: the reference's
-   ls_chpl_matrix_vector_product with device pointers.  accumulate = 1 keeps
-   y as the base of the sum when there are no diagonal terms. */
+/* On-the-fly product, FP64, real x / y on the device: the reference's
+   ls_chpl_matrix_vector_product with device pointers, in its push form
+   (each alpha_i adds Re(c) into y at index(rep beta),
+   DistributedMatrixVector.chpl:91-143), for any operator, Hermitian or
+   not.  accumulate = 1 keeps y as the base of the sum when there are no
+   diagonal terms. */
 int ls_amd_matvec_f64(ls_amd_operator *op, double const *d_x, double *d_y,
                       int accumulate);
 /* Complex x / y (interleaved re, im), y = M x with the full complex matrix
@@ -282,7 +295,9 @@ int ls_amd_matvec_f64(ls_amd_operator *op, double const *d_x, double *d_y,
 int ls_amd_matvec_c128(ls_amd_operator *op, double const *d_x, double *d_y);
 
 /* CSR cache: off-diagonal part as rows of (column, value) plus a dense
-   diagonal.  precision: 64 stores FP64 values, 32 stores FP32 values, 96
+   diagonal; row j holds every (i, C) that alpha_i emits with target
+   index(rep beta) = j, in ascending i, so the CSR products compute the same
+   push-form y = M x as the reference for any operator.  precision: 64 stores FP64 values, 32 stores FP32 values, 96
    stores both; 128 stores complex values M[i, j] as (re, im) pairs in FP64
    and FP32, for sector batches (the real-valued CSR products reject it).
    Returns the number of stored off-diagonal nonzeros or -1. */
@@ -311,6 +326,11 @@ int ls_amd_csr_matvec_f64_block(ls_amd_operator *op, int k, double const *d_x,
 /* Choose the CSR kernel schedules by timing the candidates on the current
    matrix; returns the chosen FP64 lanes per row. */
 int ls_amd_operator_autotune(ls_amd_operator *op);
+/* Whether the product of a mode (0 FP64 CSR, 1 FP32 CSR, 2 on-the-fly,
+   3 on-the-fly complex) satisfies <u, M v> = <M u, v> on seeded vectors:
+   1 yes (symmetric Re(M) for real modes, Hermitian M for mode 3), 0 no,
+   -1 on error.  The Lanczos solvers refuse operators that fail it. */
+int ls_amd_operator_is_hermitian(ls_amd_operator *op, int mode);
 /* Current CSR product schedule: out[5] = lanes per row (FP64), flags (FP64),
    lanes per row (FP32 block), flags (FP32 block), persistent grid size.
    flags: bit 0 non-temporal loads of the CSR stream, bit 1 die-aware

@@ -70,17 +70,127 @@ int number_bits_of(ls_hs_basis const *b) {
 
 // ---------------------------------------------------------------------------
 // Operators used through ls_chpl_matrix_vector_product are cached per
-// ls_hs_operator, keyed by the terms and the representatives they act on.
+// ls_hs_operator.  An entry is valid only for the exact data it was built
+// from: the basis, its representatives (address, count and a sample of
+// their contents), the serial numbers of the engine's group and index data
+// behind the basis kernels, and the terms' contents.  Destroying a basis's
+// group or index data evicts every entry built on it, and the device memory
+// the entries hold is capped (least recently used entries go first).  A
+// product holds its entry through a shared_ptr, so eviction during the
+// product cannot free what it uses.
 // ---------------------------------------------------------------------------
+struct CacheSignature {
+  ls_hs_basis const *basis = nullptr;
+  void const *reps = nullptr;
+  uint64_t count = 0;
+  uint64_t reps_sample = 0;
+  uint64_t group_serial = 0;
+  uint64_t index_serial = 0;
+  uint64_t terms = 0;
+  bool operator==(CacheSignature const &o) const {
+    return basis == o.basis && reps == o.reps && count == o.count && reps_sample == o.reps_sample &&
+           group_serial == o.group_serial && index_serial == o.index_serial && terms == o.terms;
+  }
+};
+
 struct CachedOperator {
   OperatorCore core;
-  uint64_t key = 0;
+  CacheSignature sig;
+  GroupData const *group = nullptr; // engine data this entry reads (for eviction)
+  IndexData const *index = nullptr;
   DeviceArray<uint64_t> owned_reps;
   IndexData owned_index;
   bool use_csr = false;
+  size_t device_bytes = 0;
+  uint64_t last_use = 0;
 };
-std::map<ls_hs_operator const *, std::unique_ptr<CachedOperator>> g_op_cache;
+std::map<ls_hs_operator const *, std::shared_ptr<CachedOperator>> g_op_cache;
 std::mutex g_op_mutex;
+uint64_t g_op_clock = 0;
+
+uint64_t fnv_bytes(uint64_t h, void const *p, size_t bytes) {
+  auto const *c = static_cast<unsigned char const *>(p);
+  for (size_t i = 0; i < bytes; ++i) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+uint64_t terms_hash(ls_hs_nonbranching_terms const *t) {
+  if (t == nullptr) return 0x1234;
+  uint64_t h = 1469598103934665603ull ^ static_cast<uint64_t>(t->number_terms);
+  h = fnv_bytes(h, t->v, sizeof(ls_hs_scalar) * t->number_terms);
+  for (uint64_t const *a : {t->m, t->r, t->x, t->s}) h = fnv_bytes(h, a, 8 * t->number_terms);
+  return h;
+}
+
+// Count plus 64 evenly spaced words of the host representatives.
+uint64_t reps_sample_hash(uint64_t const *reps, uint64_t count) {
+  uint64_t h = fnv_bytes(1469598103934665603ull, &count, sizeof(count));
+  if (reps == nullptr || count == 0) return h;
+  for (uint64_t q = 0; q < 64; ++q) {
+    uint64_t const i = count == 1 ? 0 : q * (count - 1) / 63;
+    h = fnv_bytes(h, &reps[i], sizeof(uint64_t));
+  }
+  return h;
+}
+
+CacheSignature signature_of(ls_hs_operator const *op) {
+  ls_hs_basis const *basis = op->basis;
+  CacheSignature s;
+  s.basis = basis;
+  s.reps = basis->representatives.elts;
+  s.count = basis->representatives.num_elts;
+  s.reps_sample = reps_sample_hash(static_cast<uint64_t const *>(s.reps), s.count);
+  if (basis->kernels != nullptr) {
+    if (GroupData const *g = as_group(basis->kernels->state_info_data)) s.group_serial = g->serial;
+    if (IndexData const *d = as_index(basis->kernels->state_index_data)) s.index_serial = d->serial;
+  }
+  s.terms = terms_hash(op->diag_terms) * 31 + terms_hash(op->off_diag_terms) * 131;
+  return s;
+}
+
+size_t csr_bytes(Csr const &c) {
+  size_t b = static_cast<size_t>(c.rows + 1) * 8 + static_cast<size_t>(c.nnz) * 4;
+  if (!c.packed) b += static_cast<size_t>(c.nnz) * (c.is_complex ? 16 : 8);
+  return b + static_cast<size_t>(c.rows) * 8;
+}
+
+// Device memory the cached entries may hold together: LSAMD_CACHE_BYTES, or
+// half of the device's memory.
+size_t cache_capacity() {
+  if (char const *e = std::getenv("LSAMD_CACHE_BYTES")) return static_cast<size_t>(std::strtoull(e, nullptr, 10));
+  size_t free_b = 0, total_b = 0;
+  (void)hipMemGetInfo(&free_b, &total_b);
+  return total_b / 2;
+}
+
+// Evicts least recently used entries (never `keep`) until the total fits.
+void enforce_cache_capacity(CachedOperator const *keep) {
+  size_t const cap = cache_capacity();
+  while (true) {
+    size_t total = 0;
+    auto victim = g_op_cache.end();
+    for (auto it = g_op_cache.begin(); it != g_op_cache.end(); ++it) {
+      total += it->second->device_bytes;
+      if (it->second.get() != keep && (victim == g_op_cache.end() || it->second->last_use < victim->second->last_use))
+        victim = it;
+    }
+    if (total <= cap || victim == g_op_cache.end()) return;
+    g_op_cache.erase(victim);
+  }
+}
+
+// Every entry that reads `group` or `index` (engine data about to be freed).
+void evict_entries_on(void const *group, void const *index) {
+  std::lock_guard<std::mutex> lock(g_op_mutex);
+  for (auto it = g_op_cache.begin(); it != g_op_cache.end();) {
+    CachedOperator const &c = *it->second;
+    bool const hit = (group != nullptr && c.group == group) || (index != nullptr && c.index == index);
+    it = hit ? g_op_cache.erase(it) : std::next(it);
+  }
+}
 
 // Fills the basis side of an OperatorCore from an ls_hs_basis whose
 // representatives are built.  `holder` keeps device copies made here.
@@ -99,6 +209,7 @@ bool core_from_basis(OperatorCore &core, ls_hs_basis const *basis, CachedOperato
   } else if (g != nullptr && g->size > 0) {
     core.kind = kPermutations;
     core.group = g;
+    holder.group = g;
   } else if (basis->spin_inversion != 0) {
     core.kind = kInversionOnly;
   } else {
@@ -109,6 +220,7 @@ bool core_from_basis(OperatorCore &core, ls_hs_basis const *basis, CachedOperato
   uint64_t const *host_reps = static_cast<uint64_t const *>(basis->representatives.elts);
   if (idx != nullptr && idx->h_reps == host_reps && idx->number_states == core.number_states) {
     core.index = idx->view();
+    holder.index = idx;
   } else {
     if (!holder.owned_reps.upload(host_reps, static_cast<size_t>(core.number_states))) return false;
     if (!index_build(holder.owned_index, holder.owned_reps.ptr, core.number_states, nbits,
@@ -119,35 +231,20 @@ bool core_from_basis(OperatorCore &core, ls_hs_basis const *basis, CachedOperato
   return true;
 }
 
-CachedOperator *cached_operator(ls_hs_operator const *op) {
-  ls_hs_basis const *basis = op->basis;
-  uint64_t key = reinterpret_cast<uintptr_t>(basis) ^ (basis->representatives.num_elts * 0x9e3779b97f4a7c15ull) ^
-                 reinterpret_cast<uintptr_t>(basis->representatives.elts);
-  // fingerprints of the host terms
-  auto fp = [](ls_hs_nonbranching_terms const *t) -> uint64_t {
-    if (t == nullptr) return 0x1234;
-    uint64_t h = 1469598103934665603ull ^ static_cast<uint64_t>(t->number_terms);
-    auto mix = [&](void const *p, size_t bytes) {
-      auto const *c = static_cast<unsigned char const *>(p);
-      for (size_t i = 0; i < bytes; ++i) {
-        h ^= c[i];
-        h *= 1099511628211ull;
-      }
-    };
-    mix(t->v, sizeof(ls_hs_scalar) * t->number_terms);
-    mix(t->m, 8 * t->number_terms);
-    mix(t->r, 8 * t->number_terms);
-    mix(t->x, 8 * t->number_terms);
-    mix(t->s, 8 * t->number_terms);
-    return h;
-  };
-  key ^= fp(op->diag_terms) * 31 + fp(op->off_diag_terms) * 131;
+std::shared_ptr<CachedOperator> cached_operator(ls_hs_operator const *op) {
+  CacheSignature const sig = signature_of(op);
   std::lock_guard<std::mutex> lock(g_op_mutex);
   auto it = g_op_cache.find(op);
-  if (it != g_op_cache.end() && it->second->key == key) return it->second.get();
-  auto c = std::make_unique<CachedOperator>();
-  c->key = key;
-  if (!core_from_basis(c->core, basis, *c)) return nullptr;
+  if (it != g_op_cache.end()) {
+    if (it->second->sig == sig) {
+      it->second->last_use = ++g_op_clock;
+      return it->second;
+    }
+    g_op_cache.erase(it);
+  }
+  auto c = std::make_shared<CachedOperator>();
+  c->sig = sig;
+  if (!core_from_basis(c->core, op->basis, *c)) return nullptr;
   if (!c->core.diag.load(op->diag_terms) || !c->core.off.load(op->off_diag_terms)) return nullptr;
   // CSR when it fits comfortably in device memory (LSAMD_MATVEC=otf|csr overrides).
   char const *mode = std::getenv("LSAMD_MATVEC");
@@ -158,10 +255,13 @@ CachedOperator *cached_operator(ls_hs_operator const *op) {
   c->use_csr = mode != nullptr ? std::strcmp(mode, "csr") == 0 : estimate < 0.4 * static_cast<double>(free_b);
   if (c->use_csr) {
     if (!operator_build_csr(c->core, 64)) return nullptr;
+    c->device_bytes = csr_bytes(c->core.csr);
   }
-  CachedOperator *raw = c.get();
-  g_op_cache[op] = std::move(c);
-  return raw;
+  c->device_bytes += static_cast<size_t>(c->core.number_states) * 8 * (c->owned_reps.ptr != nullptr ? 2 : 1);
+  c->last_use = ++g_op_clock;
+  g_op_cache[op] = c;
+  enforce_cache_capacity(c.get());
+  return c;
 }
 
 void free_external(void *p) { std::free(p); }
@@ -327,7 +427,10 @@ ls_hs_create_state_index_binary_search_kernel_data(chpl_external_array const *re
 }
 
 extern "C" void ls_hs_destroy_state_index_binary_search_kernel_data(ls_hs_state_index_binary_search_data *cache) {
-  delete as_index(cache);
+  IndexData *d = as_index(cache);
+  if (d == nullptr) return;
+  evict_entries_on(nullptr, d);
+  delete d;
 }
 
 extern "C" void ls_hs_state_index_binary_search_kernel(ptrdiff_t batch_size, uint64_t const *spins,
@@ -356,7 +459,12 @@ extern "C" ls_internal_halide_kernel_data *ls_internal_create_halide_kernel_data
   return reinterpret_cast<ls_internal_halide_kernel_data *>(d);
 }
 
-extern "C" void ls_internal_destroy_halide_kernel_data(ls_internal_halide_kernel_data *p) { delete as_group(p); }
+extern "C" void ls_internal_destroy_halide_kernel_data(ls_internal_halide_kernel_data *p) {
+  GroupData *g = as_group(p);
+  if (g == nullptr) return;
+  evict_entries_on(g, nullptr);
+  delete g;
+}
 
 extern "C" void ls_hs_is_representative_halide_kernel(ptrdiff_t batch_size, uint64_t const *alphas,
                                                       ptrdiff_t alphas_stride, uint8_t *are_representatives,
@@ -494,7 +602,7 @@ extern "C" void ls_chpl_matrix_vector_product(ls_hs_operator *matrixPtr, int num
     raise_last();
     return;
   }
-  CachedOperator *c = cached_operator(matrixPtr);
+  std::shared_ptr<CachedOperator> c = cached_operator(matrixPtr);
   if (c == nullptr) {
     raise_last();
     return;
@@ -526,6 +634,16 @@ extern "C" void ls_chpl_init(void) {
 extern "C" void ls_chpl_finalize(void) {
   std::lock_guard<std::mutex> lock(g_op_mutex);
   g_op_cache.clear();
+}
+
+extern "C" void ls_amd_forget_operator(ls_hs_operator const *op) {
+  std::lock_guard<std::mutex> lock(g_op_mutex);
+  g_op_cache.erase(op);
+}
+
+extern "C" int64_t ls_amd_operator_cache_entries(void) {
+  std::lock_guard<std::mutex> lock(g_op_mutex);
+  return static_cast<int64_t>(g_op_cache.size());
 }
 
 // ===========================================================================
@@ -777,6 +895,17 @@ extern "C" int ls_amd_csr_matvec_f64_block(ls_amd_operator *op, int k, double co
 }
 
 extern "C" int ls_amd_operator_autotune(ls_amd_operator *op) { return operator_autotune(op->core); }
+
+extern "C" int ls_amd_operator_is_hermitian(ls_amd_operator *op, int mode) {
+  if (op == nullptr) return fail("null argument") ? 0 : -1;
+  if (mode < 0 || mode > 3) return fail("mode must be 0, 1, 2 or 3") ? 0 : -1;
+  Csr const &c = op->core.csr;
+  if ((mode == 0 && !c.has64) || (mode == 1 && !c.has32)) return fail("build the CSR in that precision first") ? 0 : -1;
+  if (mode < 2 && c.is_complex) return fail("the CSR holds complex values: use mode 3") ? 0 : -1;
+  bool h = false;
+  if (!operator_is_hermitian(op->core, mode, h)) return -1;
+  return h ? 1 : 0;
+}
 
 extern "C" int ls_amd_operator_set_csr_packing(ls_amd_operator *op, int enable) {
   if (op == nullptr) return fail("null argument") ? 0 : -1;

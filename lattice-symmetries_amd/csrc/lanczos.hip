@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "operator_internal.hpp"
@@ -393,6 +394,70 @@ bool rayleigh_residual(OperatorCore &op, double const *d_u, double &theta, doubl
   return true;
 }
 
+// Seeded test of <u, M v> = <M u, v> through a product y = M x on FP64
+// vectors of n_reals doubles (interleaved re, im when complex_values): the
+// relative asymmetry |<u, M v> - <M u, v>| / (|u| |M v| + |M u| |v|), which
+// is rounding-sized for a symmetric (real) or Hermitian (complex) M and of
+// order 1 / sqrt(n) or more otherwise.
+bool asymmetry_test(int64_t n_reals, bool complex_values, std::function<bool(double const *, double *)> const &prod,
+                    uint64_t seed, double &rel) {
+  rel = 0.0;
+  if (n_reals <= 0) return true;
+  DeviceArray<double> u, v, Mu, Mv, t;
+  Blas blas;
+  if (!u.allocate(n_reals) || !v.allocate(n_reals) || !Mu.allocate(n_reals) || !Mv.allocate(n_reals) ||
+      !t.allocate(n_reals) || !blas.init(n_reals, 1))
+    return false;
+  int const g = grid_for(n_reals, kBlock);
+  random_normal_kernel<<<g, kBlock, 0, stream()>>>(seed, u.ptr, n_reals);
+  random_normal_kernel<<<g, kBlock, 0, stream()>>>(seed ^ 0x9e3779b97f4a7c15ull, v.ptr, n_reals);
+  LSAMD_HIP(hipGetLastError());
+  if (!prod(u.ptr, Mu.ptr) || !prod(v.ptr, Mv.ptr)) return false;
+  double a_re = 0, b_re = 0, a_im = 0, b_im = 0, uu = 0, vv = 0, mumu = 0, mvmv = 0;
+  if (!blas.dot(u.ptr, Mv.ptr, a_re) || !blas.dot(Mu.ptr, v.ptr, b_re)) return false;
+  if (complex_values) {
+    // u . (i w) = -Im(u^H w) for interleaved complex vectors
+    times_i_kernel<<<grid_for(n_reals / 2, kBlock), kBlock, 0, stream()>>>(Mv.ptr, t.ptr, n_reals / 2);
+    if (!blas.dot(u.ptr, t.ptr, a_im)) return false;
+    times_i_kernel<<<grid_for(n_reals / 2, kBlock), kBlock, 0, stream()>>>(v.ptr, t.ptr, n_reals / 2);
+    if (!blas.dot(Mu.ptr, t.ptr, b_im)) return false;
+  }
+  if (!blas.dot(u.ptr, u.ptr, uu) || !blas.dot(v.ptr, v.ptr, vv) || !blas.dot(Mu.ptr, Mu.ptr, mumu) ||
+      !blas.dot(Mv.ptr, Mv.ptr, mvmv))
+    return false;
+  double const scale = std::sqrt(uu * mvmv) + std::sqrt(mumu * vv);
+  double const diff = std::hypot(a_re - b_re, a_im - b_im);
+  rel = scale > 0.0 ? diff / scale : 0.0;
+  return true;
+}
+
+// Whether the product of `mode` (0 FP64 CSR, 1 FP32 CSR, 2 on-the-fly,
+// 3 on-the-fly complex) is symmetric (real modes) or Hermitian (mode 3);
+// the answer is kept on the operator.
+bool operator_is_hermitian(OperatorCore &op, int mode, bool &hermitian) {
+  bool const cplx = mode == 3;
+  int &known = op.hermitian[cplx ? 1 : 0];
+  if (known < 0) {
+    Product p;
+    if (!p.init(op, mode)) return false;
+    double rel = 0.0;
+    auto prod = [&](double const *x, double *y) { return p.apply(x, y); };
+    if (!asymmetry_test(vector_length(op, mode), cplx, prod, 0x4e524d4cull, rel)) return false;
+    known = rel <= (mode == 1 ? 1e-5 : 1e-9) ? 1 : 0;
+  }
+  hermitian = known == 1;
+  return true;
+}
+
+bool require_hermitian(OperatorCore &op, int mode) {
+  bool h = false;
+  if (!operator_is_hermitian(op, mode, h)) return false;
+  if (!h)
+    return fail(std::string("the operator is not ") + (mode == 3 ? "Hermitian" : "symmetric") +
+                ": the Lanczos solvers need <u, H v> = <H u, v>");
+  return true;
+}
+
 } // namespace lsamd
 
 using namespace lsamd;
@@ -418,6 +483,7 @@ extern "C" int ls_amd_lanczos(ls_amd_operator *op, int mode, int max_steps, int 
     return fail("build the CSR cache in the requested precision first") ? 0 : -1;
   if ((mode == 0 || mode == 1) && op->core.csr.is_complex)
     return fail("the CSR holds complex values: use mode 3") ? 0 : -1;
+  if (!require_hermitian(op->core, mode)) return -1;
   auto t0 = Clock::now();
   int64_t const n = vector_length(op->core, mode);
   DeviceArray<double> u;
@@ -488,6 +554,7 @@ extern "C" int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_
   if (c.is_complex) return fail("the CSR holds complex values: use a sector batch") ? 0 : -1;
   if (!c.has64 || (mode == 1 && !c.has32))
     return fail("build the CSR cache in both precisions first (precision 96)") ? 0 : -1;
+  if (!require_hermitian(op->core, 0)) return -1;
   BlockKrylovOptions o;
   o.block = block_size;
   o.nev = nev;
@@ -521,6 +588,7 @@ extern "C" int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_siz
   if ((mode == 0 && !op->core.csr.has64) || (mode == 1 && !op->core.csr.has32))
     return fail("build the CSR cache in the requested precision first") ? 0 : -1;
   if (!op->core.csr.has64) return fail("the FP64 CSR is needed for the final FP64 cycles (precision 96)") ? 0 : -1;
+  if (!require_hermitian(op->core, 0)) return -1;
   auto t0 = Clock::now();
   int64_t const n = op->core.number_states;
   BlockKrylovOptions o;

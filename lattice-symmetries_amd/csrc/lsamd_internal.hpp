@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <memory>
@@ -150,9 +151,17 @@ struct TermsView {
 constexpr uint32_t kGroupMagic = 0x4c534147u; // "LSAG"
 constexpr uint32_t kIndexMagic = 0x4c534149u; // "LSAI"
 
+// A process-wide serial number for host objects, so that an object created
+// at the address of a destroyed one is still told apart from it.
+inline uint64_t next_object_serial() {
+  static std::atomic<uint64_t> counter{0};
+  return ++counter;
+}
+
 // Private data behind ls_internal_create_halide_kernel_data.
 struct GroupData {
   uint32_t magic = kGroupMagic;
+  uint64_t serial = next_object_serial();
   int number_bits = 0;
   int size = 0;
   int depth = 0;
@@ -187,6 +196,7 @@ std::vector<int> group_rejection_order(GroupData const &g, int hamming, int numb
 // Private data behind ls_hs_create_state_index_binary_search_kernel_data.
 struct IndexData {
   uint32_t magic = kIndexMagic;
+  uint64_t serial = next_object_serial();
   int64_t number_states = 0;
   uint64_t const *h_reps = nullptr; // borrowed host list (may be null)
   int number_bits = 0;

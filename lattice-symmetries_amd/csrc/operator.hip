@@ -4,13 +4,17 @@
 // (beta, c) pairs, c is rescaled by chi_beta n_beta / n_alpha after
 // state_info, and y[index(rep beta)] += Re(c) x_i with relaxed atomics
 // (chapel/src/BatchedOperator.chpl:207-253,
-// chapel/src/DistributedMatrixVector.chpl:91-143).
-//
-// On the GPU the same matrix is applied in pull form: row i gathers
-// conj(C) x_j over the pairs C = chi_beta v (-1)^.. n_beta / n_alpha that
-// alpha_i itself emits.  For a Hermitian operator M[i, j] = conj(M[j, i]),
-// so every row is computed by one lane without atomics, and for real x the
-// reference's real cast gives the same Re(C) in both forms.
+// chapel/src/DistributedMatrixVector.chpl:91-143).  The engine computes the
+// same matrix M, M[index(rep beta), i] = C, for every operator, Hermitian or
+// not:
+// - the fused on-the-fly product is push form too: one lane per alpha_i,
+//   the diagonal pass first, then atomic adds of Re(c) (or c x_i for complex
+//   vectors) into y;
+// - the CSR build emits the pairs of each alpha_i as a source row, then
+//   transposes them on the device (count -> scan -> scatter -> per-row sort),
+//   so that row j of the cached matrix holds every (i, C) with target j, in
+//   ascending i.  Products then gather x along a row and write y once,
+//   without atomics, and every build stores the same bits.
 #include <algorithm>
 #include <cstring>
 
@@ -213,15 +217,19 @@ __device__ __forceinline__ int next_applicable(TermsLds const &l, int T, uint64_
   return k;
 }
 
-// Coefficient C of term k for row alpha (pull form, without x): returns the
-// representative of beta in `beta` and C in (c_re, c_im).
+// Coefficient of term k for alpha with the factor `scale` (x_alpha, or 1):
+// returns the representative of beta in `beta` and
+// c = chi_beta v (-1)^popcount(alpha & s) scale n_beta / n_alpha in
+// (c_re, c_im), with the reference's order of operations
+// (reference.c:97-134 multiplies v by sign * x; BatchedOperator.chpl:241-247
+// scales by n_beta / n_alpha and then multiplies by chi).
 template <typename W, int D, int SI, int KIND>
 __device__ __forceinline__ void off_diag_entry(GroupView const &g, OpView const &op,
                                                TermsLds const &l, int k, uint64_t alpha,
-                                               double n_alpha, uint64_t &beta, double &c_re,
-                                               double &c_im) {
-  double const sign = (__popcll(alpha & l.s[k]) & 1) ? -1.0 : 1.0;
-  double re = l.v[2 * k] * sign, im = l.v[2 * k + 1] * sign;
+                                               double n_alpha, double scale, uint64_t &beta,
+                                               double &c_re, double &c_im) {
+  double const factor = ((__popcll(alpha & l.s[k]) & 1) ? -1.0 : 1.0) * scale;
+  double re = l.v[2 * k] * factor, im = l.v[2 * k + 1] * factor;
   beta = alpha ^ l.x[k];
   if constexpr (KIND == kInversionOnly) {
     uint64_t const inv = beta ^ op.flip; // BatchedOperator.chpl:192-199
@@ -245,65 +253,79 @@ __device__ __forceinline__ void record_error(int *err, uint64_t *err_state, uint
   if (atomicCAS(err, 0, 1) == 0) *err_state = beta;
 }
 
+// Diagonal pass of the fused product (DistributedMatrixVector.chpl:34-71):
+// y_i = sum_k Re(v_k) (-1)^.. x_i in the term order of reference.c:67-95;
+// without diagonal terms y is zeroed, or left as it is when accumulating.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void otf_diag_kernel(OpView op, double const *__restrict__ x,
+                                                       double *__restrict__ y, int accumulate) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= op.n) return;
+  if (op.diag.count == 0) {
+    if (!accumulate) {
+      if constexpr (CPLX) {
+        y[2 * i] = 0.0;
+        y[2 * i + 1] = 0.0;
+      } else {
+        y[i] = 0.0;
+      }
+    }
+    return;
+  }
+  uint64_t const alpha = op.idx.reps[i];
+  double const xr = CPLX ? x[2 * i] : x[i];
+  double const xi = CPLX ? x[2 * i + 1] : 0.0;
+  double base_re = 0, base_im = 0;
+  for (int k = 0; k < op.diag.count; ++k) {
+    if ((alpha & op.diag.m[k]) != op.diag.r[k]) continue;
+    double const sign = (__popcll(alpha & op.diag.s[k]) & 1) ? -1.0 : 1.0;
+    base_re += op.diag.v[2 * k] * (sign * xr);
+    if constexpr (CPLX) base_im += op.diag.v[2 * k] * (sign * xi);
+  }
+  if constexpr (CPLX) {
+    y[2 * i] = base_re;
+    y[2 * i + 1] = base_im;
+  } else {
+    y[i] = base_re;
+  }
+}
+
+// Off-diagonal pass, push form (DistributedMatrixVector.chpl:91-143): lane i
+// emits the pairs of alpha_i and adds them into y at index(rep beta).  Real
+// vectors follow the reference exactly: c includes x_i, and a pair whose
+// real part is 0 is skipped before the index lookup.  Complex vectors add
+// c x_i for every nonzero c.
 template <typename W, int D, int SI, int KIND, bool CPLX>
-__global__ __launch_bounds__(256) void otf_kernel(GroupView g, OpView op,
-                                                  double const *__restrict__ x,
-                                                  double *__restrict__ y, int accumulate,
-                                                  int *err, uint64_t *err_state) {
+__global__ __launch_bounds__(256) void otf_push_kernel(GroupView g, OpView op, double const *__restrict__ x,
+                                                       double *__restrict__ y, int *err, uint64_t *err_state) {
   extern __shared__ uint64_t smem[];
   TermsLds const l = stage_terms(op.off, smem);
   int const T = op.off.count;
   int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool const valid = i < op.n;
   uint64_t const alpha = valid ? op.idx.reps[i] : 0;
-  double base_re = 0, base_im = 0;
-  if (valid) {
-    double const xr = CPLX ? x[2 * i] : x[i];
-    double const xi = CPLX ? x[2 * i + 1] : 0.0;
-    if (op.diag.count > 0) {
-      // reference.c:67-95 term order; Re(v) only
-      for (int k = 0; k < op.diag.count; ++k) {
-        if ((alpha & op.diag.m[k]) != op.diag.r[k]) continue;
-        double const sign = (__popcll(alpha & op.diag.s[k]) & 1) ? -1.0 : 1.0;
-        base_re += op.diag.v[2 * k] * (sign * xr);
-        if constexpr (CPLX) base_im += op.diag.v[2 * k] * (sign * xi);
-      }
-    } else if (accumulate) {
-      base_re = CPLX ? y[2 * i] : y[i];
-      base_im = CPLX ? y[2 * i + 1] : 0.0;
-    }
-  }
+  double const xr = valid ? (CPLX ? x[2 * i] : x[i]) : 0.0;
+  double const xi = (CPLX && valid) ? x[2 * i + 1] : 0.0;
   double const n_alpha = (KIND == kPermutations && valid) ? op.norms[i] : 1.0;
-  double acc_re = 0, acc_im = 0;
   int k = valid ? next_applicable(l, T, alpha, 0) : T;
   while (__any(k < T)) {
     if (k < T) {
       uint64_t beta;
       double c_re, c_im;
-      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, beta, c_re, c_im);
+      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, CPLX ? 1.0 : xr, beta, c_re, c_im);
       bool const nz = CPLX ? (c_re != 0 || c_im != 0) : (c_re != 0);
       if (nz) {
         int64_t const j = op.identity_index ? static_cast<int64_t>(beta) : index_lookup(op.idx, beta);
         if (j < 0 || j >= op.n) {
           record_error(err, err_state, beta);
         } else if constexpr (CPLX) {
-          double const xr = x[2 * j], xi = x[2 * j + 1];
-          // conj(C) * x_j
-          acc_re += c_re * xr + c_im * xi;
-          acc_im += c_re * xi - c_im * xr;
+          unsafeAtomicAdd(&y[2 * j], c_re * xr - c_im * xi);
+          unsafeAtomicAdd(&y[2 * j + 1], c_re * xi + c_im * xr);
         } else {
-          acc_re += c_re * x[j];
+          unsafeAtomicAdd(&y[j], c_re);
         }
       }
       k = next_applicable(l, T, alpha, k + 1);
-    }
-  }
-  if (valid) {
-    if constexpr (CPLX) {
-      y[2 * i] = base_re + acc_re;
-      y[2 * i + 1] = base_im + acc_im;
-    } else {
-      y[i] = base_re + acc_re;
     }
   }
 }
@@ -376,13 +398,18 @@ bool operator_matvec_otf(OperatorCore &op, double const *d_x, double *d_y, bool 
   GroupView const g = group_view_or_empty(op);
   size_t const smem = terms_lds_bytes(op.off.count);
   timer_begin(kTimerOtfF64);
-  bool ok = dispatch_op(op, [&]<typename W, int D, int SI, int KIND>() -> bool {
+  if (complex_values)
+    otf_diag_kernel<true><<<grid_for(n, 256), 256, 0, stream()>>>(v, d_x, d_y, accumulate ? 1 : 0);
+  else
+    otf_diag_kernel<false><<<grid_for(n, 256), 256, 0, stream()>>>(v, d_x, d_y, accumulate ? 1 : 0);
+  LSAMD_HIP(hipGetLastError());
+  bool ok = op.off.count == 0 || dispatch_op(op, [&]<typename W, int D, int SI, int KIND>() -> bool {
     if (complex_values)
-      otf_kernel<W, D, SI, KIND, true><<<grid_for(n, 256), 256, smem, stream()>>>(
-          g, v, d_x, d_y, accumulate ? 1 : 0, op.d_err.ptr, op.d_err_state.ptr);
+      otf_push_kernel<W, D, SI, KIND, true><<<grid_for(n, 256), 256, smem, stream()>>>(
+          g, v, d_x, d_y, op.d_err.ptr, op.d_err_state.ptr);
     else
-      otf_kernel<W, D, SI, KIND, false><<<grid_for(n, 256), 256, smem, stream()>>>(
-          g, v, d_x, d_y, accumulate ? 1 : 0, op.d_err.ptr, op.d_err_state.ptr);
+      otf_push_kernel<W, D, SI, KIND, false><<<grid_for(n, 256), 256, smem, stream()>>>(
+          g, v, d_x, d_y, op.d_err.ptr, op.d_err_state.ptr);
     LSAMD_HIP(hipGetLastError());
     return true;
   });
@@ -428,7 +455,7 @@ __global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
     if (k < T) {
       uint64_t beta;
       double c_re, c_im;
-      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, beta, c_re, c_im);
+      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, 1.0, beta, c_re, c_im);
       uint32_t col = kDropped;
       double val_re = 0.0, val_im = 0.0;
       if (CPLX ? (c_re != 0 || c_im != 0) : (c_re != 0)) {
@@ -438,7 +465,7 @@ __global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
         } else {
           col = static_cast<uint32_t>(j);
           val_re = c_re;
-          val_im = -c_im; // pull form: M[i, j] = conj(C)
+          val_im = c_im;
           ++nkept;
         }
       }
@@ -471,6 +498,157 @@ __global__ __launch_bounds__(256) void csr_compact_kernel(int64_t n, int64_t con
     for (int q = 0; q < width; ++q) vals[o * width + q] = old_vals[k * width + q];
     ++o;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Source rows -> target rows.  Entry t of source row i (column j, value C)
+// becomes an entry of row j with column i.  Keys i << 24 | t are scattered
+// into the target rows through atomic cursors, each target row's keys are
+// sorted, and the sorted keys give the final columns and values: row j then
+// lists its sources in ascending i (in term order within one i), whatever
+// order the atomics took.
+// ---------------------------------------------------------------------------
+
+constexpr int kKeyLocalBits = 24; // entries per source row < 2^24 (one per term at most)
+constexpr int kSortCap = 1024;    // longest row sorted in LDS; longer rows are heap-sorted in place
+
+__global__ __launch_bounds__(256) void push_count_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                         uint32_t const *__restrict__ cols,
+                                                         uint32_t *__restrict__ counts) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) atomicAdd(&counts[cols[p]], 1u);
+}
+
+__global__ __launch_bounds__(256) void push_scatter_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                           uint32_t const *__restrict__ cols,
+                                                           int64_t const *__restrict__ target_ptr,
+                                                           uint32_t *__restrict__ cursor,
+                                                           uint64_t *__restrict__ keys) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t const b = rp[i], e = rp[i + 1];
+  for (int64_t p = b; p < e; ++p) {
+    uint32_t const j = cols[p];
+    uint32_t const slot = atomicAdd(&cursor[j], 1u);
+    keys[target_ptr[j] + slot] = (static_cast<uint64_t>(i) << kKeyLocalBits) | static_cast<uint64_t>(p - b);
+  }
+}
+
+__device__ void heap_sort(uint64_t *a, int64_t n) {
+  auto sift = [a](int64_t root, int64_t end) {
+    while (true) {
+      int64_t child = 2 * root + 1;
+      if (child >= end) return;
+      if (child + 1 < end && a[child] < a[child + 1]) ++child;
+      if (a[root] >= a[child]) return;
+      uint64_t const t = a[root];
+      a[root] = a[child];
+      a[child] = t;
+      root = child;
+    }
+  };
+  for (int64_t r = n / 2 - 1; r >= 0; --r) sift(r, n);
+  for (int64_t end = n - 1; end > 0; --end) {
+    uint64_t const t = a[0];
+    a[0] = a[end];
+    a[end] = t;
+    sift(0, end);
+  }
+}
+
+// One wave per row: bitonic sort of the row's keys in LDS, padded to a power
+// of two with all-ones keys.  The row index is uniform over the workgroup,
+// so every barrier is reached by all its threads.
+__global__ __launch_bounds__(64) void sort_rows_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                       uint64_t *__restrict__ keys) {
+  __shared__ uint64_t s[kSortCap];
+  for (int64_t row = blockIdx.x; row < n; row += gridDim.x) {
+    int64_t const b = rp[row];
+    int64_t const len = rp[row + 1] - b;
+    if (len <= 1) continue;
+    if (len > kSortCap) {
+      if (threadIdx.x == 0) heap_sort(keys + b, len);
+      continue;
+    }
+    int P = 2;
+    while (P < len) P <<= 1;
+    for (int q = threadIdx.x; q < P; q += 64) s[q] = q < len ? keys[b + q] : ~uint64_t{0};
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int q = threadIdx.x; q < P / 2; q += 64) {
+          int const lo = 2 * stride * (q / stride) + (q % stride);
+          int const hi = lo + stride;
+          bool const ascending = (lo & size) == 0;
+          uint64_t const a = s[lo], c = s[hi];
+          if ((a > c) == ascending) {
+            s[lo] = c;
+            s[hi] = a;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int q = threadIdx.x; q < len; q += 64) keys[b + q] = s[q];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void push_finalize_kernel(int64_t nnz, uint64_t const *__restrict__ keys,
+                                                            int64_t const *__restrict__ src_ptr,
+                                                            double const *__restrict__ src_vals, int width,
+                                                            uint32_t *__restrict__ cols,
+                                                            double *__restrict__ vals) {
+  int64_t const q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= nnz) return;
+  uint64_t const key = keys[q];
+  int64_t const i = static_cast<int64_t>(key >> kKeyLocalBits);
+  int64_t const p = src_ptr[i] + static_cast<int64_t>(key & ((uint64_t{1} << kKeyLocalBits) - 1));
+  cols[q] = static_cast<uint32_t>(i);
+  for (int w = 0; w < width; ++w) vals[q * width + w] = src_vals[p * width + w];
+}
+
+// Replaces the source-row CSR (row_ptr, cols, vals of `width` doubles per
+// entry, nnz entries over n rows) by its target-row form.
+bool transpose_to_targets(int64_t n, int64_t nnz, int width, DeviceArray<int64_t> &row_ptr,
+                          DeviceArray<uint32_t> &cols, DeviceArray<double> &vals) {
+  if (n == 0 || nnz == 0) return true;
+  DeviceArray<uint32_t> counts;
+  DeviceArray<int64_t> target_ptr;
+  DeviceArray<uint64_t> keys;
+  if (!counts.allocate(static_cast<size_t>(n)) || !target_ptr.allocate(static_cast<size_t>(n + 1)) ||
+      !keys.allocate(static_cast<size_t>(nnz)))
+    return false;
+  LSAMD_HIP(hipMemsetAsync(counts.ptr, 0, sizeof(uint32_t) * n, stream()));
+  push_count_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, counts.ptr);
+  LSAMD_HIP(hipGetLastError());
+  uint64_t total = 0;
+  if (!exclusive_scan(counts.ptr, n, reinterpret_cast<uint64_t *>(target_ptr.ptr), total)) return false;
+  if (static_cast<int64_t>(total) != nnz) return fail("CSR transpose: entry count mismatch");
+  int64_t const last = nnz;
+  LSAMD_HIP(hipMemcpyAsync(target_ptr.ptr + n, &last, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+  LSAMD_HIP(hipMemsetAsync(counts.ptr, 0, sizeof(uint32_t) * n, stream()));
+  push_scatter_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, target_ptr.ptr,
+                                                              counts.ptr, keys.ptr);
+  LSAMD_HIP(hipGetLastError());
+  int const sort_grid = static_cast<int>(std::min<int64_t>(n, 1 << 16));
+  sort_rows_kernel<<<sort_grid, 64, 0, stream()>>>(n, target_ptr.ptr, keys.ptr);
+  LSAMD_HIP(hipGetLastError());
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  counts.reset();
+  cols.reset();
+  DeviceArray<uint32_t> cols2;
+  DeviceArray<double> vals2;
+  if (!cols2.allocate(static_cast<size_t>(nnz)) || !vals2.allocate(static_cast<size_t>(nnz) * width)) return false;
+  push_finalize_kernel<<<grid_for(nnz, 256), 256, 0, stream()>>>(nnz, keys.ptr, row_ptr.ptr, vals.ptr, width,
+                                                                 cols2.ptr, vals2.ptr);
+  LSAMD_HIP(hipGetLastError());
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  row_ptr = std::move(target_ptr);
+  cols = std::move(cols2);
+  vals = std::move(vals2);
+  return true;
 }
 
 __global__ __launch_bounds__(256) void to_f32_kernel(int64_t n, double const *__restrict__ in,
@@ -607,6 +785,7 @@ bool operator_build_csr(OperatorCore &op, int precision) {
   bool const cplx = precision == 128;
   int const width = cplx ? 2 : 1;
   if (op.number_states >= (int64_t{1} << 32) - 1) return fail("CSR columns are 32-bit: too many states");
+  if (op.off.count >= (1 << kKeyLocalBits)) return fail("CSR build: too many off-diagonal terms");
   if (!ensure_err(op) || !operator_prepare_norms(op)) return false;
   int64_t const n = op.number_states;
   Csr &c = op.csr;
@@ -682,6 +861,8 @@ bool operator_build_csr(OperatorCore &op, int precision) {
     vals = std::move(vals2);
   }
   c.nnz = static_cast<int64_t>(kept);
+  // Source rows (the pairs each alpha_i emits) -> rows of M.
+  if (!transpose_to_targets(n, c.nnz, width, row_ptr, cols, vals)) return false;
   c.row_ptr = std::move(row_ptr);
   c.cols = std::move(cols);
   if (cplx) {

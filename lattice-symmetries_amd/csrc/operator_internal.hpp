@@ -71,6 +71,8 @@ struct OperatorCore {
   bool allow_packing = true;    // value-table CSR when possible
   DeviceArray<int> d_err; // error flag + offending state
   DeviceArray<uint64_t> d_err_state;
+  // <u, M v> = <M u, v> for real / complex vectors: -1 untested, 0 no, 1 yes
+  int hermitian[2] = {-1, -1};
 };
 
 bool operator_prepare_norms(OperatorCore &op);
@@ -81,6 +83,13 @@ bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool acc
 bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_y);
 bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *d_y);
 int operator_autotune(OperatorCore &op);
+
+// Relative asymmetry of a product on seeded vectors (lanczos.hip), and the
+// operator's cached answer for a product mode (0 FP64 CSR, 1 FP32 CSR,
+// 2 on-the-fly, 3 on-the-fly complex).
+bool asymmetry_test(int64_t n_reals, bool complex_values, std::function<bool(double const *, double *)> const &prod,
+                    uint64_t seed, double &rel);
+bool operator_is_hermitian(OperatorCore &op, int mode, bool &hermitian);
 
 // Term application on device arrays (reference.c:67-134).
 bool apply_diag_device(Terms const &t, int64_t count, uint64_t const *d_alphas, double *d_ys,

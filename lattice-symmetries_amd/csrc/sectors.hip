@@ -47,6 +47,7 @@ struct ls_amd_sector_batch {
   };
   std::vector<Graph> graphs;
   uint64_t graph_clock = 0;
+  int hermitian = -1; // every sector's M Hermitian: -1 untested, 0 no, 1 yes
   static void release(Graph &g) {
     if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
     if (g.graph != nullptr) (void)hipGraphDestroy(g.graph);
@@ -467,6 +468,19 @@ bool dense_sector(ls_amd_sector_batch const &bt, int s, int nev, double *evals, 
 // and left out of the block Lanczos process.
 bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &opts, BlockKrylovResult &res) {
   auto const t_start = std::chrono::steady_clock::now();
+  if (bt.hermitian < 0) {
+    // one complex column per state: the product's [re | im] rows interleave
+    auto prod = [&](double const *x, double *y) -> bool {
+      if (!launch_product(bt, 0, 1, x, y, bt.lanes, bt.blocks, bt.xcd_aware)) return false;
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    };
+    double rel = 0.0;
+    if (!asymmetry_test(2 * bt.csr.rows, true, prod, 0x53454354ull, rel)) return false;
+    bt.hermitian = rel <= 1e-9 ? 1 : 0;
+  }
+  if (bt.hermitian == 0)
+    return fail("a sector's operator is not Hermitian: the block Lanczos solver needs <u, H v> = <H u, v>");
   BlockKrylovOptions o = opts;
   o.complex_values = true;
   int const ns = bt.count;

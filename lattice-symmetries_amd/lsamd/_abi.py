@@ -185,6 +185,8 @@ SIGNATURES = {
     ),
     "ls_chpl_init": (None, []),
     "ls_chpl_finalize": (None, []),
+    "ls_amd_forget_operator": (None, [C.POINTER(ls_hs_operator)]),
+    "ls_amd_operator_cache_entries": (C.c_int64, []),
     "ls_chpl_enumerate_representatives": (None, [C.POINTER(ls_hs_basis), C.c_uint64, C.c_uint64, C.POINTER(chpl_external_array)]),
     "ls_chpl_operator_apply_diag": (None, [C.POINTER(ls_hs_operator), C.c_int64, _u64p, C.POINTER(chpl_external_array), C.c_int64]),
     "ls_chpl_operator_apply_off_diag": (
@@ -231,6 +233,7 @@ SIGNATURES = {
     "ls_amd_csr_matvec_f32": (C.c_int, [_P, C.c_int, _P, _P]),
     "ls_amd_csr_matvec_f64_block": (C.c_int, [_P, C.c_int, _P, _P]),
     "ls_amd_operator_autotune": (C.c_int, [_P]),
+    "ls_amd_operator_is_hermitian": (C.c_int, [_P, C.c_int]),
     "ls_amd_operator_set_csr_packing": (C.c_int, [_P, C.c_int]),
     "ls_amd_operator_csr_info": (C.c_int, [_P, _i64p]),
     "ls_amd_operator_schedule": (C.c_int, [_P, C.POINTER(C.c_int)]),

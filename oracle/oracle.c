@@ -890,23 +890,29 @@ ORA_EXPORT int64_t ora_matrix_coo(ora_group const *g, int lanes, int kind,
 }
 
 /*
- * CPU baseline: the push-form product restricted to rows [row0, row1) of
- * the basis, OpenMP-parallel with relaxed atomic adds into y like
- * ConcurrentAccessor.chpl:31-33.  Returns the number of nonzero
- * contributions processed (diagonal entries included).
+ * CPU baseline: the push-form product from the source rows row0, row0 +
+ * stride, ... (< row1) of the basis, OpenMP-parallel with relaxed atomic adds
+ * into y like ConcurrentAccessor.chpl:31-33: the diagonal pass assigns y at
+ * those rows (DistributedMatrixVector.chpl:34-71), then each source row adds
+ * its off-diagonal pairs wherever they land (y must hold zeros elsewhere).
+ * Returns the number of nonzero contributions processed (diagonal entries
+ * included).  stride 1 over [0, num_states) is the whole product.
  */
 ORA_EXPORT int64_t ora_matvec_push_parallel(
     ora_group const *g, int lanes, int64_t num_states, uint64_t const *reps,
-    int64_t row0, int64_t row1, int Td, double const *vd, uint64_t const *md,
-    uint64_t const *rd, uint64_t const *sd, int To, double const *vo,
-    uint64_t const *mo, uint64_t const *ro, uint64_t const *xo,
-    uint64_t const *so, double const *x, double *y, int num_threads) {
+    int64_t row0, int64_t row1, int64_t stride, int Td, double const *vd,
+    uint64_t const *md, uint64_t const *rd, uint64_t const *sd, int To,
+    double const *vo, uint64_t const *mo, uint64_t const *ro,
+    uint64_t const *xo, uint64_t const *so, double const *x, double *y,
+    int num_threads) {
   int64_t nnz = 0;
   if (row0 < 0) row0 = 0;
   if (row1 > num_states) row1 = num_states;
-  /* Diagonal pass first, assigning y (DistributedMatrixVector.chpl:34-71). */
+  if (stride < 1) stride = 1;
+  int64_t const count = row1 > row0 ? (row1 - row0 + stride - 1) / stride : 0;
 #pragma omp parallel for schedule(static) reduction(+ : nnz) num_threads(num_threads)
-  for (int64_t i = row0; i < row1; ++i) {
+  for (int64_t q = 0; q < count; ++q) {
+    int64_t const i = row0 + q * stride;
     uint64_t const a = reps[i];
     double d = 0;
     for (int t = 0; t < Td; ++t)
@@ -918,7 +924,8 @@ ORA_EXPORT int64_t ora_matvec_push_parallel(
     ++nnz;
   }
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : nnz) num_threads(num_threads)
-  for (int64_t i = row0; i < row1; ++i) {
+  for (int64_t q = 0; q < count; ++q) {
+    int64_t const i = row0 + q * stride;
     uint64_t const a = reps[i];
     uint64_t rr;
     double ch[2], na;
@@ -949,9 +956,12 @@ ORA_EXPORT int64_t ora_matvec_push_parallel(
 /*
  * Selected rows of y = Re(M) x, each computed from the pairs that its own
  * representative emits: y_i = d_i x_i + sum Re(C) x_idx(rep beta), with C as
- * in BatchedOperator.chpl:232-247.  Equal to the push form because Re(M) is
- * symmetric (checked on dense matrices by the tests); used to spot-check
- * rows of full-size products.  Returns -1 on a state outside the basis.
+ * in BatchedOperator.chpl:232-247.  This is row i of Re(M)^T x: equal to the
+ * push form (row i of Re(M) x) only when Re(M) is symmetric, which holds for
+ * Hermitian operators that respect the group (the tests pin the equality on
+ * such sectors and show the difference on an operator that is not).  Used to
+ * spot-check rows of full-size products.  Returns -1 on a state outside the
+ * basis.
  */
 ORA_EXPORT int ora_matvec_rows(ora_group const *g, int lanes, int64_t num_states,
                                uint64_t const *reps, int64_t nrows,

@@ -109,7 +109,7 @@ def lib():
                                      C.c_int, _f64p, _u64p, _u64p, _u64p, _u64p, C.c_int64, _i64p, _i64p,
                                      _f64p, _f64p]
         L.ora_matvec_push_parallel.restype = C.c_int64
-        L.ora_matvec_push_parallel.argtypes = [C.c_void_p, C.c_int, C.c_int64, _u64p, C.c_int64, C.c_int64,
+        L.ora_matvec_push_parallel.argtypes = [C.c_void_p, C.c_int, C.c_int64, _u64p, C.c_int64, C.c_int64, C.c_int64,
                                                C.c_int, _f64p, _u64p, _u64p, _u64p,
                                                C.c_int, _f64p, _u64p, _u64p, _u64p, _u64p, _f64p, _f64p, C.c_int]
         L.ora_num_threads.restype = C.c_int
@@ -336,9 +336,12 @@ def matrix_dense(group, number_sites, spin_inversion, reps, diag: Terms, off: Te
     return M if complex_values else M.real
 
 
-def matvec_push_parallel(group, reps, diag: Terms, off: Terms, x, y, row0, row1, threads):
+def matvec_push_parallel(group, reps, diag: Terms, off: Terms, x, y, row0, row1, threads, stride=1):
+    """Push-form product from source rows row0, row0 + stride, ... < row1
+    (OpenMP, atomic adds into y, which must be zero elsewhere); returns the
+    number of contributions."""
     reps = np.ascontiguousarray(reps, dtype=np.uint64)
-    return int(lib().ora_matvec_push_parallel(group.h, 8, reps.size, _p(reps, _u64p), row0, row1,
+    return int(lib().ora_matvec_push_parallel(group.h, 8, reps.size, _p(reps, _u64p), row0, row1, stride,
                                               *diag.args(with_x=False), *off.args(), _p(x, _f64p),
                                               _p(y, _f64p), threads))
 

@@ -186,6 +186,10 @@ def test_chain28_sector_batch_full_size(engine):
     assert batch.count == 56 and batch.number_states == 40116600
     sizes = np.diff(batch.offsets)
     assert sizes.min() == 716014 and sizes.max() == 716728
+    # every sector's dimension is its Burnside count (cycle formula, no enumeration)
+    from test_dimensions import dim
+    want = [dim(n, h, [models.chain_translation(n)], [k], si) for k, si in batch.sectors]
+    assert [int(d) for d in sizes] == want
     names = [f"{k}{'+' if si == 1 else '-'}" for k, si in batch.sectors]
     refs = {nm: ops[names.index(nm)] for nm in ("5+", "11-")}
     del bases, ops
