@@ -273,6 +273,20 @@ int ls_amd_basis_is_representative_device(ls_amd_basis const *b, int64_t count,
 int ls_amd_basis_index_device(ls_amd_basis const *b, int64_t count,
                               uint64_t const *d_states, int64_t *d_indices);
 
+/* state_info without device allocations.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+   arguments and results as ls_hs_state_info_halide_kernel (host arrays,
+   strides in elements, private_data from
+   ls_internal_create_halide_kernel_data), but the device staging lives in a
+   fixed 2.5 MiB array of the library, so a call takes at most 65536 states
+   (larger batches return -1 with ls_amd_last_error set).  Calls in one
+   process share that array and are serialised.  Returns 0 or -1. */
+int ls_amd_state_info_noalloc(ptrdiff_t batch_size, uint64_t const *alphas,
+                              ptrdiff_t alphas_stride, uint64_t *betas,
+                              ptrdiff_t betas_stride, ls_hs_scalar *characters,
+                              double *norms, void const *private_data);
+
 /* Operator handle over an ls_amd_basis: diagonal and off-diagonal
    non-branching terms (one 64-bit word per mask, as the reference's
    matvec requires, BatchedOperator.chpl:327-328). */

@@ -222,6 +222,10 @@ SIGNATURES = {
     "ls_amd_basis_state_info_device": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P]),
     "ls_amd_basis_is_representative_device": (C.c_int, [_P, C.c_int64, _P, _P, _P]),
     "ls_amd_basis_index_device": (C.c_int, [_P, C.c_int64, _P, _P]),
+    "ls_amd_state_info_noalloc": (
+        C.c_int,
+        [C.c_ssize_t, _u64p, C.c_ssize_t, _u64p, C.c_ssize_t, C.POINTER(ls_hs_scalar), _f64p, _P],
+    ),
     "ls_amd_operator_create": (_P, [_P, C.POINTER(ls_hs_nonbranching_terms), C.POINTER(ls_hs_nonbranching_terms)]),
     "ls_amd_operator_destroy": (None, [_P]),
     "ls_amd_matvec_f64": (C.c_int, [_P, _P, _P, C.c_int]),
