@@ -35,13 +35,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_KIND = ("L2 (TCC) miss bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known "
+                "size; what leaves the L2 for the fabric (Infinity Cache or HBM), an upper bound on HBM bytes")
+
+
 def traffic_rate(traffic, kernel_ms):
-    """The measured HBM bytes per launch over the measured kernel time: the
+    """The measured L2-miss bytes per launch over the measured kernel time: the
     rate the memory system actually sustains, beside the algorithmic one."""
     if traffic is None or not kernel_ms:
-        return {"traffic_GBs": None, "traffic_frac": None}
+        return {"traffic_kind": TRAFFIC_KIND, "traffic_GBs": None, "traffic_frac": None}
     gbs = traffic / (kernel_ms * 1e-3) / 1e9
-    return {"traffic_GBs": gbs, "traffic_frac": gbs / HBM_PEAK_GBS}
+    return {"traffic_kind": TRAFFIC_KIND, "traffic_GBs": gbs, "traffic_frac": gbs / HBM_PEAK_GBS}
 
 
 def profiled_traffic(workload, kernel):
@@ -67,29 +71,57 @@ def workload(L=6, J2=0.5):
     return n, n // 2, S, expr
 
 
-def cpu_baseline(S, n, reps, expr, seconds):
-    """Oracle push-form product (the reference's algorithm: Benes orbit
-    search per beta, binary-search index, atomic accumulate) on a sample of
-    rows, timed on the host cores."""
+def cpu_baseline(S, n, h, reps, expr, seconds):
+    """The oracle (a C/OpenMP restatement of the reference's CPU algorithm) on
+    every host core this process may use, on bounded samples of the workload:
+    - the push-form product (Benes orbit search per beta, binary-search
+      index, atomic accumulate) from source rows spread evenly over the whole
+      basis (every stride-th representative);
+    - the basis enumeration (Gosper steps + orbit filter) over eight candidate
+      ranges spread evenly over the combinadics rank range."""
     from oracle import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = len(os.sched_getaffinity(0))
     g = O.Group(n, S.elements(), S.characters.real, S.characters.imag, 1, masks=S.masks, shifts=S.shifts)
     g.use_benes(True)
     diag, off = expr.split()
     d, o = O.Terms(diag), O.Terms(off)
     x = np.random.default_rng(42).standard_normal(reps.size)
     y = np.zeros(reps.size)
-    rows = min(2000, reps.size)
+    stride = max(1, reps.size // 4000)
     t0 = time.perf_counter()
-    nnz = O.matvec_push_parallel(g, reps, d, o, x, y, 0, rows, threads)
+    O.matvec_push_parallel(g, reps, d, o, x, y, 0, reps.size, threads, stride=stride)
     dt = time.perf_counter() - t0
-    rows = int(min(reps.size, max(rows, rows * seconds / max(dt, 1e-3))))
+    per_row = dt / ((reps.size + stride - 1) // stride)
+    stride = max(1, int(per_row * reps.size / (0.7 * seconds)))
+    y[:] = 0
     t0 = time.perf_counter()
-    nnz = O.matvec_push_parallel(g, reps, d, o, x, y, 0, rows, threads)
+    nnz = O.matvec_push_parallel(g, reps, d, o, x, y, 0, reps.size, threads, stride=stride)
     dt = time.perf_counter() - t0
+    rows = (reps.size + stride - 1) // stride
+    # enumeration: eight equal candidate ranges spread over the rank range
+    lo = (1 << h) - 1
+    hi = lo << (n - h - 1)
+    r0, r1 = O.state_to_index(lo), O.state_to_index(hi)
+    total = r1 - r0 + 1
+    chunk = min(1 << 20, total)
+    t0 = time.perf_counter()
+    g.count_parallel(O.index_to_state(r0, h), chunk, threads)
+    t_probe = time.perf_counter() - t0
+    parts = 8 if total >= 8 else 1
+    chunk = int(min(total // parts, max(1 << 16, chunk * 0.3 * seconds / parts / max(t_probe, 1e-3))))
+    found = 0
+    t0 = time.perf_counter()
+    for q in range(parts):
+        start = r0 + (total - chunk) * q // max(parts - 1, 1)
+        found += g.count_parallel(O.index_to_state(start, h), chunk, threads)
+    t_enum = time.perf_counter() - t0
     return {"value": nnz / dt, "unit": "nonzeros/s", "cores": threads, "kind": "port",
-            "sample": f"push-form H x over the first {rows} of {reps.size} representatives "
-                      f"({nnz} nonzeros, {dt:.1f} s, Benes orbit search, {threads} OpenMP threads)"}
+            "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "sample": f"push-form H x from every {stride}-th of {reps.size} representatives ({rows} source rows, "
+                      f"{nnz} nonzeros, {dt:.1f} s, Benes orbit search, {threads} OpenMP threads)",
+            "basis": {"candidates_per_s": parts * chunk / t_enum, "states_per_s": found / t_enum,
+                      "sample": f"enumeration of {parts} ranges of {chunk} candidates spread over the {total} "
+                                f"candidates ({found} representatives, {t_enum:.1f} s, {threads} OpenMP threads)"}}
 
 
 def sectors_bench(args):
@@ -132,8 +164,13 @@ def sectors_bench(args):
     t_steps = time.perf_counter() - t0
     ms = float(np.mean(k_ms))
     nnz = nnz_off + dim
-    bytes_per_launch = 76.0 * nnz_off + 140.0 * dim
+    # compulsory bytes: the CSR stream (column 4 + complex FP32 value 8 per
+    # nonzero), row pointer 8 and diagonal 4 per row, x read once and y
+    # written once (8 complex FP32 columns, 64 B each per row)
+    bytes_per_launch = 12.0 * nnz_off + 140.0 * dim
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9
+    # the per-nonzero model that charges every x-row gather (76 B per nonzero)
+    gather_model = (76.0 * nnz_off + 140.0 * dim) / (ms * 1e-3) / 1e9
     sector_kernel = (f"sector_spmv_kernel<8, {sched['lanes']}, float, float>"
                      f"{' die-aware' if sched['die_aware'] else ''} x{sched['blocks']}")
     traffic, traffic_src = profiled_traffic(f"chain{n}_sectors", sector_kernel)
@@ -176,7 +213,9 @@ def sectors_bench(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      **traffic_rate(traffic, ms),
-                     "kernel": sector_kernel, "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "kernel": sector_kernel, "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "bytes_model": "compulsory: CSR stream once, x read once, y written once",
+                     "gather_model_GBs": gather_model},
         "setup_seconds": t_setup,
         "eigensolve": {"method": "complex thick-restart block Lanczos, FP32 then FP64",
                        "block_complex_columns": 4, "schedule": sched4, "max_blocks": args.sector_blocks,
@@ -210,6 +249,53 @@ def timed(lib, family, steps, launch):
         ms.append(lib.ls_amd_last_kernel_ms(family))
     lib.ls_amd_synchronize()
     return float(np.mean(ms))
+
+
+def fp64_product(lib, op, dim, steps, x, y):
+    """Autotunes and times the FP64 CSR product of `op` (built) against the
+    per-nonzero model of the headline (20 B per stored nonzero, 32 B per row)."""
+    op.autotune()
+    sched = op.schedule
+    info = op.csr_info
+    ms = timed(lib, 0, steps, lambda: op.csr_matvec(x, y))
+    nnz_off = info["nnz"]
+    achieved = (20.0 * nnz_off + 32.0 * dim) / (ms * 1e-3) / 1e9
+    kernel = f"csr_f64_kernel<{sched['lanes_f64']}, {sched_suffix(sched, info['packed'])}>"
+    if sched["flags_f64"] & 6 == 6:
+        kernel += " full grid"
+    return {"kernel": kernel, "kernel_ms": ms, "off_diagonal_nonzeros": nnz_off, "packed": info["packed"],
+            "value_table": info["table_size"] if info["packed"] else None, "schedule": sched,
+            "nonzeros_per_s": (nnz_off + dim) / (ms * 1e-3), "achieved_GBs": achieved,
+            "frac": achieved / HBM_PEAK_GBS}
+
+
+def general_case(lib, basis, L, dim, steps, rng):
+    """The FP64 product where the value table does not apply: the same 6x6
+    matrix stored unpacked (column and FP64 value per entry), and a model
+    with its own coupling on every bond (too many distinct values to pack)."""
+    import lsamd
+    from lsamd import models
+    x = lsamd.DeviceArray.from_numpy(rng.standard_normal(dim))
+    y = lsamd.DeviceArray(8 * dim)
+    out = {}
+    op = lsamd.DeviceOperator(basis, models.j1j2_square(L, 1.0, 0.5))
+    op.set_csr_packing(False)
+    t0 = time.perf_counter()
+    op.build_csr(64)
+    out["fp64_unpacked"] = {"csr_build_seconds": time.perf_counter() - t0,
+                            **fp64_product(lib, op, dim, steps, x, y)}
+    del op
+    op = lsamd.DeviceOperator(basis, models.random_bond_square(L, seed=7))
+    t0 = time.perf_counter()
+    op.build_csr(64)
+    out["fp64_random_couplings"] = {
+        "model": f"J1-J2 bonds of the {L}x{L} lattice with seeded couplings J (1 + 0.5 u), u in [-1/2, 1/2) per "
+                 "bond (seed 7), on the same symmetric basis (push-form product of the reference's formula)",
+        "csr_build_seconds": time.perf_counter() - t0, **fp64_product(lib, op, dim, steps, x, y)}
+    del op
+    for k, v in out.items():
+        log(f"{k}: {v['kernel_ms']:.3f} ms, {v['frac']:.3f} of the roofline, packed {v['packed']}")
+    return out
 
 
 def configs_bench(args):
@@ -306,19 +392,29 @@ def configs_bench(args):
         del op, basis
 
 
+def probes():
+    """libprobes.so: measurement kernels built beside the engine (not part of it)."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(REPO, "lattice-symmetries_amd", "libprobes.so"))
+    lib.lsamd_probe_error.restype = C.c_char_p
+    lib.lsamd_probe_gather.argtypes = [C.c_int64, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(C.c_double)]
+    lib.lsamd_probe_copy.argtypes = [C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    return lib
+
+
 def copy_calibration(args):
-    """A plain device copy of known size (torch's copy kernel), run under the
-    profiler's FETCH_SIZE / WRITE_SIZE counters to scale them to bytes."""
-    import torch
+    """A plain device copy of known size, run under the profiler's FETCH_SIZE /
+    WRITE_SIZE counters to scale them to bytes."""
+    import ctypes as C
+    lib = probes()
     nbytes = args.copy_mib << 20
-    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    for _ in range(args.steps):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"workload": "copy-calibration", "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
-                      "copies": args.steps}), flush=True)
+    ms = C.c_double()
+    if lib.lsamd_probe_copy(nbytes, args.steps, C.byref(ms)) != 0:
+        raise SystemExit(lib.lsamd_probe_error().decode())
+    print(json.dumps({"workload": "copy-calibration", "kernel": "copy_kernel", "bytes_read_per_copy": nbytes,
+                      "bytes_written_per_copy": nbytes, "copies": args.steps, "ms_per_copy": ms.value,
+                      "GBs": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
 
 
 def gather_ceiling(args):
@@ -329,8 +425,7 @@ def gather_ceiling(args):
     any reordering of the real matrix could reach with one gather per
     nonzero."""
     import ctypes as C
-    import lsamd
-    lib = lsamd.load()
+    lib = probes()
     rows, per_row = 15804956, 74
     out = []
     for packed in (1, 0):
@@ -338,9 +433,9 @@ def gather_ceiling(args):
             for flags in (0, 6):  # plain grid; die-aware mapping, one workgroup per tile
                 for lanes in (8, 16, 32):
                     ms = C.c_double()
-                    rc = lib.ls_amd_gather_probe(rows, per_row, window, lanes, flags, packed, 5, C.byref(ms))
+                    rc = lib.lsamd_probe_gather(rows, per_row, window, lanes, flags, packed, 5, C.byref(ms))
                     if rc != 0:
-                        raise SystemExit(lsamd._abi.last_error())
+                        raise SystemExit(lib.lsamd_probe_error().decode())
                     nnz = rows * per_row
                     out.append({"packed": packed, "window_columns": window, "window_MB": window * 8 / 1e6,
                                 "flags": flags, "lanes": lanes, "ms": ms.value,
@@ -360,6 +455,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lattice", type=int, default=6)
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-general", action="store_true", help="skip the unpacked and random-coupling products")
     ap.add_argument("--no-reference-abi", action="store_true",
                     help="skip timing ls_chpl_matrix_vector_product with host vectors")
     ap.add_argument("--lanczos-steps", type=int, default=200)
@@ -385,6 +481,9 @@ def main():
     if args.workload == "copy-calibration":
         copy_calibration(args)
         return
+    if args.workload == "gather-ceiling":
+        gather_ceiling(args)
+        return
     import lsamd
     from lsamd import _abi
     lib = lsamd.load()
@@ -392,9 +491,6 @@ def main():
         raise SystemExit("no usable gfx950 device: " + _abi.last_error())
     if args.workload == "chain-sectors":
         sectors_bench(args)
-        return
-    if args.workload == "gather-ceiling":
-        gather_ceiling(args)
         return
     if args.workload == "configs":
         configs_bench(args)
@@ -464,8 +560,13 @@ def main():
     lib.ls_amd_synchronize()
     t_b = time.perf_counter() - t0
     block_rate = 8.0 * nnz * args.steps / t_b
-    block_bytes = 40.0 * nnz_off + dim * (8 + 4 + 64)
-    block_gbs = block_bytes / (float(np.mean(b_ms)) * 1e-3) / 1e9
+    b_ms_mean = float(np.mean(b_ms))
+    # compulsory bytes: the stored CSR stream (4 B per nonzero with the value
+    # table, else 8), row pointer 8 and diagonal 4 per row, x read once and y
+    # written once (8 FP32 columns: 32 B each per row)
+    block_bytes = (4.0 if packed else 8.0) * nnz_off + dim * (8 + 4 + 64)
+    block_gbs = block_bytes / (b_ms_mean * 1e-3) / 1e9
+    block_gather_gbs = (40.0 * nnz_off + dim * (8 + 4 + 64)) / (b_ms_mean * 1e-3) / 1e9
 
     # ---- the reference-named entry point: host x and y, copied over PCIe on every call ----
     ref_abi = None
@@ -493,6 +594,9 @@ def main():
         log(f"reference entry point: {1e3 * t_call:.1f} ms per call with host x and y "
             f"({nnz / t_call:.3e} nonzeros/s), first call {t_first:.1f} s")
         del hop, sb, dxh
+
+    # ---- the general case: no value table ----
+    general = None if args.no_general else general_case(lib, basis, args.lattice, dim, args.steps, rng)
 
     # ---- on-the-fly fused FP64 (no CSR) ----
     t0 = time.perf_counter()
@@ -546,13 +650,16 @@ def main():
         "csr_build_seconds": t_csr,
         "reference_abi_host_vectors": ref_abi,
         "fp32_block8": {"nonzero_rhs_per_s": block_rate, "speedup_vs_fp64": block_rate / value,
-                        "kernel_ms": float(np.mean(b_ms)), "achieved_GBs": block_gbs},
+                        "kernel_ms": b_ms_mean, "achieved_GBs": block_gbs, "frac": block_gbs / HBM_PEAK_GBS,
+                        "bytes_model": "compulsory: CSR stream once, x read once, y written once",
+                        "gather_model_GBs": block_gather_gbs},
+        "general_case": general,
         "otf_fp64": {"seconds": t_otf, "nonzeros_per_s": nnz / t_otf},
         "ground_state": e0,
     }
     if not args.no_cpu_baseline:
         states = basis.states
-        line["cpu_baseline"] = cpu_baseline(S, n, states, expr, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(S, n, h, states, expr, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

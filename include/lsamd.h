@@ -321,6 +321,16 @@ int64_t ls_amd_operator_build_csr(ls_amd_operator *op, int precision);
    column | value index << 24, and the values in a table (set before
    ls_amd_operator_build_csr). */
 int ls_amd_operator_set_csr_packing(ls_amd_operator *op, int enable);
+/* Orbit search of the fused kernels (CSR build and on-the-fly product).
+   When the group is the set of translations of an lx x ly torus (site
+   r * lx + c) times the elements fixing site 0, and every character is real
+   +-1, each point element's image is translated by word rotations instead
+   of running one network per element: the same representatives, norms and
+   coefficients (the stored CSR is bit-identical).  On by default;
+   ls_amd_operator_set_factored(op, 0) keeps the element-by-element search,
+   ls_amd_operator_uses_factored reports which one runs (1 / 0). */
+int ls_amd_operator_set_factored(ls_amd_operator *op, int enable);
+int ls_amd_operator_uses_factored(ls_amd_operator const *op);
 /* out[4]: stored off-diagonal nonzeros, packed (0/1), value-table size,
    complex (0/1). */
 int ls_amd_operator_csr_info(ls_amd_operator const *op, int64_t *out);
@@ -524,15 +534,6 @@ ls_amd_basis *ls_amd_model_create_basis(ls_amd_model const *model);
 ls_amd_operator *ls_amd_model_create_operator(ls_amd_model const *model,
                                               int which,
                                               ls_amd_basis const *basis);
-
-/* Measurement helper (no reference counterpart): times the FP64 CSR product
-   kernel on a synthetic matrix of `rows` rows x `per_row` entries whose
-   rows gather from windows of `window` columns (window = rows: the whole
-   vector), plain (packed = 0) or value-table format, with `lanes` per row
-   and the schedule `flags` of ls_amd_operator_set_schedule.  ms_out: mean
-   duration of `reps` launches. */
-int ls_amd_gather_probe(int64_t rows, int per_row, int64_t window, int lanes,
-                        int flags, int packed, int reps, double *ms_out);
 
 /* Timing of the last launched kernel of the given family, in milliseconds
    (HIP events on the engine's stream). family: 0 csr f64, 1 csr f32 block,

@@ -913,6 +913,18 @@ extern "C" int ls_amd_operator_set_csr_packing(ls_amd_operator *op, int enable) 
   return 0;
 }
 
+extern "C" int ls_amd_operator_set_factored(ls_amd_operator *op, int enable) {
+  if (op == nullptr) return fail("null argument") ? 0 : -1;
+  op->core.allow_factored = enable != 0;
+  return 0;
+}
+
+extern "C" int ls_amd_operator_uses_factored(ls_amd_operator const *op) {
+  if (op == nullptr) return fail("null argument") ? 0 : -1;
+  OperatorCore const &c = op->core;
+  return (c.kind == kPermutations && c.allow_factored && c.group->factored()) ? 1 : 0;
+}
+
 extern "C" int ls_amd_operator_csr_info(ls_amd_operator const *op, int64_t *out) {
   if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
   Csr const &c = op->core.csr;

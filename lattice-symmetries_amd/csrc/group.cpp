@@ -226,7 +226,93 @@ GroupData *group_data_create(ls_hs_permutation_group const *g, int spin_inversio
   }
   if (!d->d_eig_re.upload(re.data(), re.size())) return nullptr;
   if (!d->d_eig_im.upload(im.data(), im.size())) return nullptr;
+  if (!group_detect_translations(*d)) return nullptr;
   return d.release();
+}
+
+bool group_detect_translations(GroupData &g) {
+  g.fact_lx = g.fact_ly = 0;
+  g.h_fact_q.clear();
+  g.h_fact_elem.clear();
+  g.h_fact_key.clear();
+  int const n = g.number_bits, G = g.size, D = g.depth;
+  if (n < 2 || G < 2) return true;
+  for (int e = 0; e < G; ++e)
+    if (g.h_eig_re[e] != 1.0 && g.h_eig_re[e] != -1.0) return true;
+  // permutations: bit i of g(x) is bit p[i] of x
+  std::vector<Perm> perms(G, Perm(n));
+  std::map<Perm, int> index;
+  for (int e = 0; e < G; ++e) {
+    std::vector<uint64_t> m(g.h_masks.begin() + static_cast<size_t>(e) * D,
+                            g.h_masks.begin() + static_cast<size_t>(e + 1) * D);
+    for (int b = 0; b < n; ++b) {
+      uint64_t const y = apply_network(m, g.h_shifts, uint64_t{1} << b);
+      if (y == 0 || (y & (y - 1)) != 0) return true;
+      perms[e][__builtin_ctzll(y)] = b;
+    }
+    index.emplace(perms[e], e);
+  }
+  if (static_cast<int>(index.size()) != G) return true;
+  auto translation = [n](int lx, int ly, int a, int b) {
+    Perm p(n);
+    for (int r = 0; r < ly; ++r)
+      for (int c = 0; c < lx; ++c) p[r * lx + c] = ((r + b) % ly) * lx + (c + a) % lx;
+    return p;
+  };
+  for (int lx = n; lx >= 1; --lx) {
+    if (n % lx != 0) continue;
+    int const ly = n / lx;
+    if (lx * ly < 2) continue;
+    bool all = true;
+    for (int b = 0; b < ly && all; ++b)
+      for (int a = 0; a < lx && all; ++a) all = index.count(translation(lx, ly, a, b)) > 0;
+    if (!all) continue;
+    std::vector<int32_t> q;
+    for (int e = 0; e < G; ++e)
+      if (perms[e][0] == 0) q.push_back(e);
+    if (static_cast<int64_t>(q.size()) * lx * ly != G) continue;
+    std::vector<int32_t> elem(static_cast<size_t>(G));
+    std::vector<char> hit(G, 0);
+    bool ok = true;
+    for (size_t k = 0; k < q.size() && ok; ++k)
+      for (int b = 0; b < ly && ok; ++b)
+        for (int a = 0; a < lx && ok; ++a) {
+          Perm const t = translation(lx, ly, a, b);
+          Perm p(n);
+          for (int i = 0; i < n; ++i) p[i] = perms[q[k]][t[i]]; // t(q(x))
+          auto it = index.find(p);
+          if (it == index.end() || hit[it->second]) {
+            ok = false;
+            break;
+          }
+          hit[it->second] = 1;
+          elem[(k * ly + b) * lx + a] = it->second;
+        }
+    if (!ok) continue;
+    uint64_t m1 = 0, m2 = 0;
+    for (int r = 0; r < ly; ++r)
+      for (int c = 0; c < lx; ++c) (c < lx - 1 ? m1 : m2) |= uint64_t{1} << (r * lx + c);
+    g.fact_lx = lx;
+    g.fact_ly = ly;
+    g.fact_m1 = m1;
+    g.fact_m2 = m2;
+    // rank of element e (unflipped; its flipped copy ranks one higher) in the
+    // eight-lane state_info reduction: lane e mod 8, then position e / 8 in
+    // the lane, then the |G| mod 8 tail elements in order
+    int const lanes = 8, chunks = G / lanes;
+    std::vector<int32_t> key(static_cast<size_t>(G));
+    for (size_t k = 0; k < elem.size(); ++k) {
+      int const e = elem[k];
+      key[k] = e < chunks * lanes ? (e % lanes) * 2 * chunks + (e / lanes) * 2
+                                  : lanes * 2 * chunks + (e - chunks * lanes) * 2;
+    }
+    g.h_fact_q = q;
+    g.h_fact_elem = elem;
+    g.h_fact_key = key;
+    return g.d_fact_q.upload(q.data(), q.size()) && g.d_fact_elem.upload(elem.data(), elem.size()) &&
+           g.d_fact_key.upload(key.data(), key.size());
+  }
+  return true;
 }
 
 // Order in which the enumeration filter tries the group elements: the

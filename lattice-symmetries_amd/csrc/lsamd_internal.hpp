@@ -122,6 +122,17 @@ struct GroupView {
   // Stabiliser sums <= norm_eps count as zero when norm_eps > 0 (0: the
   // reference's rule, norm > 0 and sqrt of the raw sum).
   double norm_eps;
+  // Translation factorisation (fact_nq = 0: none): every element is t o q
+  // for a translation t of the fact_lx x fact_ly torus (site r * lx + c) and
+  // a point element q; fact_q[q] is q's element index,
+  // fact_elem[(q * ly + b) * lx + a] the index of t_{a,b} o q and fact_key
+  // its rank in the state_info reduction order.  fact_m1 / fact_m2 are the
+  // bits with column < lx - 1 / column lx - 1.
+  int fact_lx, fact_ly, fact_nq;
+  int32_t const *fact_q;
+  int32_t const *fact_elem;
+  int32_t const *fact_key;
+  uint64_t fact_m1, fact_m2;
 };
 
 // Sorted representatives plus a prefix table over their top bits:
@@ -180,11 +191,27 @@ struct GroupData {
   DeviceArray<uint64_t> d_shifts;
   DeviceArray<double> d_eig_re, d_eig_im;
   DeviceArray<uint8_t> d_image;
+  // Translation factorisation (found by group_detect_translations; fact_lx
+  // = 0 when the group is not a torus translation group times point
+  // elements with real characters +-1).
+  int fact_lx = 0, fact_ly = 0;
+  std::vector<int32_t> h_fact_q, h_fact_elem, h_fact_key;
+  DeviceArray<int32_t> d_fact_q, d_fact_elem, d_fact_key;
+  uint64_t fact_m1 = 0, fact_m2 = 0;
+  bool factored() const { return fact_lx > 0 && fermion_species == 0; }
   GroupView view() const {
     return GroupView{d_masks.ptr, d_eig_re.ptr, d_eig_im.ptr, d_shifts.ptr, size, depth,
-                     flip_mask, d_image.ptr, number_bits, norm_eps};
+                     flip_mask, d_image.ptr, number_bits, norm_eps,
+                     fact_lx, fact_ly, static_cast<int>(h_fact_q.size()), d_fact_q.ptr, d_fact_elem.ptr,
+                     d_fact_key.ptr, fact_m1, fact_m2};
   }
 };
+
+// Looks for the factorisation G = {t o q}: t runs over the translations of
+// an lx x ly torus, q over the elements fixing site 0, and every character
+// has real part exactly +-1 (then stabiliser sums are exact in any order).
+// Leaves fact_lx = 0 when the group has no such form.
+bool group_detect_translations(GroupData &g);
 
 // Marks a group as fermionic (species 1 or 2) and uploads the orbital
 // images; spin inversion must be off.

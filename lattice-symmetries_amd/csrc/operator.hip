@@ -238,8 +238,9 @@ __device__ __forceinline__ void off_diag_entry(GroupView const &g, OpView const 
       re *= static_cast<double>(SI);
       im *= static_cast<double>(SI);
     }
-  } else if constexpr (KIND == kPermutations) {
-    StateInfo const s = state_info_one<W, D, SI>(g, beta); // BatchedOperator.chpl:232-247
+  } else if constexpr (KIND == kPermutations || KIND == kFactored) {
+    // BatchedOperator.chpl:232-247
+    StateInfo const s = KIND == kFactored ? state_info_factored<W, D, SI>(g, beta) : state_info_one<W, D, SI>(g, beta);
     double const t_re = (re * s.norm) / n_alpha, t_im = (im * s.norm) / n_alpha;
     re = s.c_re * t_re - s.c_im * t_im;
     im = s.c_re * t_im + s.c_im * t_re;
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void otf_push_kernel(GroupView g, OpView op, d
   uint64_t const alpha = valid ? op.idx.reps[i] : 0;
   double const xr = valid ? (CPLX ? x[2 * i] : x[i]) : 0.0;
   double const xi = (CPLX && valid) ? x[2 * i + 1] : 0.0;
-  double const n_alpha = (KIND == kPermutations && valid) ? op.norms[i] : 1.0;
+  double const n_alpha = ((KIND == kPermutations || KIND == kFactored) && valid) ? op.norms[i] : 1.0;
   int k = valid ? next_applicable(l, T, alpha, 0) : T;
   while (__any(k < T)) {
     if (k < T) {
@@ -334,7 +335,11 @@ __global__ __launch_bounds__(256) void otf_push_kernel(GroupView g, OpView op, d
 // the others run on 64-bit words with the spin inversion as SI.
 template <typename F> bool dispatch_op(OperatorCore const &op, F &&f) {
   if (op.kind == kPermutations) {
+    bool const factored = op.allow_factored && op.group->factored();
     return dispatch_group(*op.group, [&]<typename W, int D, int SI>() -> bool {
+      if constexpr (SI != kFermionic) {
+        if (factored) return f.template operator()<W, D, SI, kFactored>();
+      }
       return f.template operator()<W, D, SI, kPermutations>();
     });
   }
@@ -447,7 +452,7 @@ __global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
     }
     diag[i] = d;
   }
-  double const n_alpha = (KIND == kPermutations && valid) ? op.norms[i] : 1.0;
+  double const n_alpha = ((KIND == kPermutations || KIND == kFactored) && valid) ? op.norms[i] : 1.0;
   int64_t o = valid ? row_ptr[i] : 0;
   uint32_t nkept = 0;
   int k = valid ? next_applicable(l, T, alpha, 0) : T;
@@ -904,179 +909,6 @@ bool operator_build_csr(OperatorCore &op, int precision) {
 // SpMV kernels: LPR lanes share one row
 // ---------------------------------------------------------------------------
 
-// Stream loads of the CSR arrays, optionally non-temporal so that the
-// stream does not displace the gathered vector from L2.
-template <bool NT, typename T> __device__ __forceinline__ T ld_stream(T const *p) {
-  if constexpr (NT) {
-    return __builtin_nontemporal_load(p);
-  } else {
-    return *p;
-  }
-}
-
-// Row tiles of 256 / LPR rows.  Die-aware (DIE): the grid is a fixed multiple
-// of 8 workgroups; workgroups are dealt to the 8 accelerator dies round-robin,
-// and those of die d walk the contiguous tile range [tiles d / 8, tiles (d+1)
-// / 8), so rows that gather from nearby columns share one die's L2.
-// Otherwise one workgroup per tile.
-template <bool DIE> __device__ __forceinline__ void tile_range(int64_t tiles, int64_t &t, int64_t &t_end,
-                                                              int64_t &t_step) {
-  if constexpr (DIE) {
-    int const die = static_cast<int>(blockIdx.x) % 8;
-    t = tiles * die / 8 + static_cast<int64_t>(blockIdx.x) / 8;
-    t_end = tiles * (die + 1) / 8;
-    t_step = static_cast<int64_t>(gridDim.x) / 8;
-  } else {
-    t = blockIdx.x;
-    t_end = t < tiles ? t + 1 : t;
-    t_step = 1;
-  }
-}
-
-// One stored entry: (column, value), from separate arrays or from a packed
-// column word and the value table in LDS.
-template <bool PACKED, bool NT, typename T>
-__device__ __forceinline__ void csr_entry(uint32_t const *__restrict__ cols, T const *__restrict__ vals,
-                                          T const *__restrict__ table, int64_t k, uint32_t &col, T &v) {
-  if constexpr (PACKED) {
-    uint32_t const p = ld_stream<NT>(cols + k);
-    col = p & kPackedColumnMask;
-    v = table[p >> 24];
-  } else {
-    col = ld_stream<NT>(cols + k);
-    v = ld_stream<NT>(vals + k);
-  }
-}
-
-// Stages the value table in LDS (a no-op for unpacked matrices).
-template <bool PACKED, typename T>
-__device__ __forceinline__ T const *stage_table(T *lds, T const *__restrict__ table, int size) {
-  if constexpr (PACKED) {
-    for (int i = threadIdx.x; i < size; i += blockDim.x) lds[i] = table[i];
-    __syncthreads();
-    return lds;
-  } else {
-    return nullptr;
-  }
-}
-
-template <int LPR, bool NT, bool DIE, bool PACKED>
-__global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *__restrict__ rp,
-                                                      uint32_t const *__restrict__ cols,
-                                                      double const *__restrict__ vals,
-                                                      double const *__restrict__ table, int table_size,
-                                                      double const *__restrict__ diag,
-                                                      double const *__restrict__ x,
-                                                      double *__restrict__ y, int accumulate,
-                                                      int has_diag) {
-  __shared__ double lds_table[PACKED ? kValueTableSize : 1];
-  double const *tab = stage_table<PACKED>(lds_table, table, table_size);
-  constexpr int kRows = 256 / LPR;
-  int64_t const tiles = (n + kRows - 1) / kRows;
-  int64_t t, t_end, t_step;
-  tile_range<DIE>(tiles, t, t_end, t_step);
-  int const sub = static_cast<int>(threadIdx.x) % LPR;
-  for (; t < t_end; t += t_step) {
-    int64_t const row = t * kRows + static_cast<int>(threadIdx.x) / LPR;
-    double acc = 0.0;
-    if (row < n) {
-      int64_t const b = rp[row], e = rp[row + 1];
-      int64_t k = b + sub;
-      // four entries per step: their gathers are in flight together; the
-      // sum keeps the entries' order
-      for (; k + 3 * LPR < e; k += 4 * LPR) {
-        uint32_t c0, c1, c2, c3;
-        double v0, v1, v2, v3;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, c0, v0);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + LPR, c1, v1);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + 2 * LPR, c2, v2);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + 3 * LPR, c3, v3);
-        double const x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-        acc = fma(v0, x0, acc);
-        acc = fma(v1, x1, acc);
-        acc = fma(v2, x2, acc);
-        acc = fma(v3, x3, acc);
-      }
-      for (; k < e; k += LPR) {
-        uint32_t col;
-        double v;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
-        acc = fma(v, x[col], acc);
-      }
-    }
-#pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-    if (row < n && sub == 0) {
-      double const base = has_diag ? diag[row] * x[row] : (accumulate ? y[row] : 0.0);
-      y[row] = base + acc;
-    }
-  }
-}
-
-template <int K, int LPR, typename T, typename TO, bool NT, bool DIE, bool PACKED>
-__global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const *__restrict__ rp,
-                                                        uint32_t const *__restrict__ cols,
-                                                        T const *__restrict__ vals,
-                                                        T const *__restrict__ table, int table_size,
-                                                        T const *__restrict__ diag,
-                                                        T const *__restrict__ x,
-                                                        TO *__restrict__ y) {
-  __shared__ T lds_table[PACKED ? kValueTableSize : 1];
-  T const *tab = stage_table<PACKED>(lds_table, table, table_size);
-  constexpr int kRows = 256 / LPR;
-  int64_t const tiles = (n + kRows - 1) / kRows;
-  int64_t t, t_end, t_step;
-  tile_range<DIE>(tiles, t, t_end, t_step);
-  int const sub = static_cast<int>(threadIdx.x) % LPR;
-  for (; t < t_end; t += t_step) {
-    int64_t const row = t * kRows + static_cast<int>(threadIdx.x) / LPR;
-    T acc[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) acc[q] = T(0);
-    if (row < n) {
-      int64_t const b = rp[row], e = rp[row + 1];
-      int64_t k = b + sub;
-      // two entries per step, their gathers in flight together, summed in order
-      for (; k + LPR < e; k += 2 * LPR) {
-        uint32_t c0, c1;
-        T v0, v1;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, c0, v0);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + LPR, c1, v1);
-        T const *x0 = x + static_cast<size_t>(c0) * K;
-        T const *x1 = x + static_cast<size_t>(c1) * K;
-        T a0[K], a1[K];
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-          a0[q] = x0[q];
-          a1[q] = x1[q];
-        }
-#pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] = fma(v0, a0[q], acc[q]);
-#pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] = fma(v1, a1[q], acc[q]);
-      }
-      for (; k < e; k += LPR) {
-        uint32_t col;
-        T v;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
-        T const *xr = x + static_cast<size_t>(col) * K;
-#pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] = fma(v, xr[q], acc[q]);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < K; ++q)
-#pragma unroll
-      for (int off = LPR / 2; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, LPR);
-    if (row < n && sub == 0) {
-      T const d = diag[row];
-#pragma unroll
-      for (int q = 0; q < K; ++q)
-        y[static_cast<size_t>(row) * K + q] = static_cast<TO>(d * x[static_cast<size_t>(row) * K + q] + acc[q]);
-    }
-  }
-}
-
 template <typename F> bool dispatch_lpr(int lpr, F &&f) {
   switch (lpr) {
   case 1: return f.template operator()<1>();
@@ -1277,92 +1109,3 @@ int operator_autotune(OperatorCore &op) {
 }
 
 } // namespace lsamd
-
-// ---------------------------------------------------------------------------
-// Gather-rate probe: the FP64 product kernel on a synthetic matrix whose
-// rows gather from a window of `window` columns (rows of one 65 536-row block
-// share a window; window = rows draws from the whole vector).  Measures how
-// fast the product can go when its gathers are served from L2 (small
-// windows) versus from memory, with everything else as in the real product.
-// ---------------------------------------------------------------------------
-
-namespace lsamd {
-namespace {
-
-__global__ __launch_bounds__(256) void probe_rows_kernel(int64_t rows, int per_row, int64_t *__restrict__ rp) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (i <= rows) rp[i] = i * per_row;
-}
-
-__global__ __launch_bounds__(256) void probe_cols_kernel(int64_t rows, int per_row, int64_t window, int packed,
-                                                         uint32_t *__restrict__ cols) {
-  int64_t const k = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (k >= rows * per_row) return;
-  uint64_t z = 0x9e3779b97f4a7c15ull * static_cast<uint64_t>(k + 1);
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-  z ^= z >> 31;
-  int64_t const row = k / per_row;
-  uint64_t const span = static_cast<uint64_t>(rows - window);
-  uint64_t const base = window >= rows ? 0 : (static_cast<uint64_t>(row / 65536) * 2654435761ull) % (span + 1);
-  uint32_t const col = static_cast<uint32_t>(base + z % static_cast<uint64_t>(window));
-  cols[k] = packed ? (col | (static_cast<uint32_t>(z >> 40) & 0x3Fu) << 24) : col;
-}
-
-} // namespace
-} // namespace lsamd
-
-extern "C" int ls_amd_gather_probe(int64_t rows, int per_row, int64_t window, int lanes, int flags, int packed,
-                                   int reps, double *ms_out) {
-  using namespace lsamd;
-  if (ms_out == nullptr || rows <= 0 || per_row <= 0 || window <= 0 || window > rows || reps <= 0)
-    return fail("ls_amd_gather_probe: bad arguments") ? 0 : -1;
-  if (flags < 0 || flags > 7 || flags == 4 || flags == 5)
-    return fail("ls_amd_gather_probe: schedule flags are 0..3, 6 or 7") ? 0 : -1;
-  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0)
-    return fail("ls_amd_gather_probe: lanes per row must be a power of two <= 64") ? 0 : -1;
-  if (rows >= (int64_t{1} << 24) && packed) return fail("ls_amd_gather_probe: packed needs rows < 2^24") ? 0 : -1;
-  if (rows >= (int64_t{1} << 32) - 1) return fail("ls_amd_gather_probe: too many rows") ? 0 : -1;
-  if (!ensure_device()) return -1;
-  OperatorCore op;
-  Csr &c = op.csr;
-  c.rows = rows;
-  c.nnz = rows * per_row;
-  DeviceArray<double> x, y;
-  if (!c.row_ptr.allocate(static_cast<size_t>(rows + 1)) || !c.cols.allocate(static_cast<size_t>(c.nnz)) ||
-      !c.diag64.allocate(static_cast<size_t>(rows)) || !x.allocate(static_cast<size_t>(rows)) ||
-      !y.allocate(static_cast<size_t>(rows)))
-    return -1;
-  probe_rows_kernel<<<grid_for(rows + 1, 256), 256, 0, stream()>>>(rows, per_row, c.row_ptr.ptr);
-  probe_cols_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(rows, per_row, window, packed, c.cols.ptr);
-  if (hipGetLastError() != hipSuccess) return fail("ls_amd_gather_probe: launch failed") ? 0 : -1;
-  if (hipMemsetAsync(x.ptr, 0, sizeof(double) * rows, stream()) != hipSuccess) return -1;
-  if (packed) {
-    std::vector<double> table(64);
-    for (int i = 0; i < 64; ++i) table[i] = 1.0 + i;
-    if (!c.table64.upload(table.data(), table.size())) return -1;
-    c.packed = true;
-    c.table_size = 64;
-  } else {
-    if (!c.vals64.allocate(static_cast<size_t>(c.nnz))) return -1;
-    if (hipMemsetAsync(c.vals64.ptr, 0, sizeof(double) * c.nnz, stream()) != hipSuccess) return -1;
-  }
-  c.has64 = true;
-  op.lanes_f64 = lanes;
-  op.flags_f64 = flags;
-  if (!operator_csr_f64(op, x.ptr, y.ptr, false)) return -1;
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
-  (void)hipEventRecord(e0, stream());
-  bool ok = true;
-  for (int r = 0; r < reps && ok; ++r) ok = operator_csr_f64(op, x.ptr, y.ptr, false);
-  (void)hipEventRecord(e1, stream());
-  (void)hipEventSynchronize(e1);
-  float ms = 0.0f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (!ok) return -1;
-  *ms_out = static_cast<double>(ms) / reps;
-  return hipStreamSynchronize(stream()) == hipSuccess ? 0 : -1;
-}

@@ -5,6 +5,7 @@
 #include <functional>
 
 #include "lsamd_internal.hpp"
+#include "spmv_kernels.hpp"
 
 namespace lsamd {
 
@@ -23,7 +24,9 @@ struct Terms {
 };
 
 // What the product has to do with every beta (BatchedOperator.chpl:264-282).
-enum ProjectionKind { kNoProjection = 0, kInversionOnly = 1, kPermutations = 2 };
+// kFactored: kPermutations through the translation factorisation of the
+// group (state_info_factored).
+enum ProjectionKind { kNoProjection = 0, kInversionOnly = 1, kPermutations = 2, kFactored = 3 };
 
 struct Csr {
   int64_t rows = 0;
@@ -44,9 +47,6 @@ struct Csr {
   DeviceArray<double> table64;
   DeviceArray<float> table32;
 };
-
-constexpr uint32_t kPackedColumnMask = (1u << 24) - 1;
-constexpr int kValueTableSize = 256;
 
 struct OperatorCore {
   // basis side
@@ -69,6 +69,7 @@ struct OperatorCore {
   int flags_f32 = 0;
   int persistent_blocks = 1024; // grid of the die-aware schedule (a multiple of 8)
   bool allow_packing = true;    // value-table CSR when possible
+  bool allow_factored = true;   // factorised orbit search when the group has one
   DeviceArray<int> d_err; // error flag + offending state
   DeviceArray<uint64_t> d_err_state;
   // <u, M v> = <M u, v> for real / complex vectors: -1 untested, 0 no, 1 yes

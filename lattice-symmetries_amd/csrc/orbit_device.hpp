@@ -200,6 +200,72 @@ __device__ __forceinline__ StateInfo state_info_one(GroupView const &g,
   return out;
 }
 
+// state_info through the translation factorisation of the group (fact_nq >
+// 0, real parts of the characters +-1, no fermionic signs): for every point
+// element q, y = q(x) by its Benes network, then the translations t_{a,b}(y)
+// of the torus by column and row rotations (a few word operations each
+// instead of a network per element).  The orbit minimum does not depend on
+// the visiting order, and the stabiliser sum of +-1 terms is exact in any
+// order.  The character is that of the minimiser state_info_one selects:
+// fact_key ranks each element (and its spin-flipped copy, key + 1) by the
+// order in which the eight-lane reduction reaches it -- lane e mod 8 first,
+// then position within the lane, the |G| mod 8 tail elements last -- and
+// among equal images the lowest rank wins, as the strict comparisons of
+// that reduction decide.  rep, character and norm equal state_info_one's
+// bit for bit.
+template <typename W, int D, int SI>
+__device__ __forceinline__ StateInfo state_info_factored(GroupView const &g, uint64_t x64) {
+  W const x = static_cast<W>(x64);
+  W const flip = static_cast<W>(g.flip);
+  W const m1 = static_cast<W>(g.fact_m1), m2 = static_cast<W>(g.fact_m2);
+  int const lx = g.fact_lx, ly = g.fact_ly, nq = g.fact_nq;
+  int const up = g.nbits - lx; // row rotation: the bottom row moves to the top
+  W best = x;
+  int best_key = -1, best_e = -1, best_s = 0;
+  double nn = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    W const y = group_apply<W, D>(g, g.fact_q[q], x);
+    size_t const base = static_cast<size_t>(q) * lx * ly;
+#pragma unroll
+    for (int s = 0; s < (SI != 0 ? 2 : 1); ++s) {
+      double const sgn = (s == 1 && SI == -1) ? -1.0 : 1.0;
+      W row = s == 1 ? static_cast<W>(y ^ flip) : y;
+      for (int b = 0; b < ly; ++b) {
+        W w = row;
+        for (int a = 0; a < lx; ++a) {
+          size_t const k = base + static_cast<size_t>(b) * lx + a;
+          int const e = g.fact_elem[k];
+          int const key = g.fact_key[k] + s;
+          if (w < best || (w == best && key < best_key)) {
+            best = w;
+            best_key = key;
+            best_e = e;
+            best_s = s;
+          }
+          if (w == x) nn = nn + sgn * g.eig_re[e];
+          w = static_cast<W>(((w >> 1) & m1) | ((w << (lx - 1)) & m2));
+        }
+        if (ly > 1) row = static_cast<W>((row >> lx) | ((row << up) & flip));
+      }
+    }
+  }
+  StateInfo out;
+  out.rep = static_cast<uint64_t>(best);
+  if (best_e < 0) {
+    out.c_re = 1.0;
+    out.c_im = 0.0;
+  } else {
+    double const cre = g.eig_re[best_e], cim = g.eig_im[best_e];
+    bool const neg = SI == -1 && best_s == 1;
+    out.c_re = neg ? -cre : cre;
+    out.c_im = neg ? -cim : cim;
+  }
+  int const G = g.size;
+  double const denom = SI == 0 ? static_cast<double>(G) : static_cast<double>(2 * G);
+  out.norm = (g.norm_eps > 0.0 && !(nn > g.norm_eps)) ? 0.0 : sqrt(nn / denom);
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // is_representative (generator.cpp:168-272)
 // ---------------------------------------------------------------------------

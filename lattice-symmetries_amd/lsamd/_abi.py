@@ -239,6 +239,8 @@ SIGNATURES = {
     "ls_amd_operator_autotune": (C.c_int, [_P]),
     "ls_amd_operator_is_hermitian": (C.c_int, [_P, C.c_int]),
     "ls_amd_operator_set_csr_packing": (C.c_int, [_P, C.c_int]),
+    "ls_amd_operator_set_factored": (C.c_int, [_P, C.c_int]),
+    "ls_amd_operator_uses_factored": (C.c_int, [_P]),
     "ls_amd_operator_csr_info": (C.c_int, [_P, _i64p]),
     "ls_amd_operator_schedule": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "ls_amd_operator_set_schedule": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -249,7 +251,6 @@ SIGNATURES = {
                                        _P, C.POINTER(ls_amd_lanczos_result)]),
     "ls_amd_dense_eigh": (C.c_int, [C.c_int, _f64p, _f64p, _f64p]),
     "ls_amd_dense_eigh_hermitian": (C.c_int, [C.c_int, _f64p, _f64p, _f64p]),
-    "ls_amd_gather_probe": (C.c_int, [C.c_int64, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, _f64p]),
     "ls_amd_model_from_yaml": (_P, [C.c_char_p]),
     "ls_amd_model_destroy": (None, [_P]),
     "ls_amd_model_basis_info": (C.c_int, [_P, C.POINTER(C.c_int)]),

@@ -90,6 +90,20 @@ def j1j2_square(L: int, J1: float = 1.0, J2: float = 0.5, scale: float = 1.0) ->
                      [heisenberg_bond(i, j, J2 * scale) for i, j in square_nnn_bonds(L)])
 
 
+def random_bond_square(L: int, seed: int = 7, J1: float = 1.0, J2: float = 0.5, spread: float = 0.5,
+                       scale: float = 1.0) -> Expr:
+    """J1-J2 bonds of the L x L lattice with seeded couplings J (1 + spread u),
+    u uniform in [-1/2, 1/2) per bond: every bond its own coupling (the
+    lattice symmetries are broken, and the stored matrix holds many more
+    distinct values than a symmetric model)."""
+    rng = np.random.default_rng(seed)
+    nn, nnn = square_nn_bonds(L), square_nnn_bonds(L)
+    u = rng.random(len(nn) + len(nnn)) - 0.5
+    return sum_exprs([heisenberg_bond(i, j, J1 * scale * (1 + spread * u[k])) for k, (i, j) in enumerate(nn)] +
+                     [heisenberg_bond(i, j, J2 * scale * (1 + spread * u[len(nn) + k]))
+                      for k, (i, j) in enumerate(nnn)])
+
+
 def square_generators(L: int):
     """Translations, both reflections and the rotation of the model files."""
     return [square_translation_x(L), square_translation_y(L), square_reflection_x(L),
