@@ -6,6 +6,7 @@
 // ls_hs_state_index_binary_search_kernel as well as the device-array
 // variants of the additive ABI.
 #include <algorithm>
+#include <cstdlib>
 
 #include "orbit_device.hpp"
 
@@ -104,9 +105,14 @@ __global__ __launch_bounds__(256) void prefix_table_kernel(uint64_t const *__res
 }
 
 int default_prefix_bits(int number_bits, int64_t count) {
-  // about four table entries per state, at least 8 and at most 26 bits
+  // LSAMD_INDEX_PREFIX_BITS (8..30) overrides; otherwise about four table
+  // entries per state, at least 8 and at most 26 bits
   int p = 8;
-  while (p < 26 && (int64_t{1} << p) < 4 * count) ++p;
+  if (char const *e = std::getenv("LSAMD_INDEX_PREFIX_BITS")) {
+    p = std::max(8, std::min(30, std::atoi(e)));
+  } else {
+    while (p < 26 && (int64_t{1} << p) < 4 * count) ++p;
+  }
   return std::min(p, std::max(number_bits, 1));
 }
 

@@ -220,29 +220,36 @@ __device__ __forceinline__ StateInfo state_info_factored(GroupView const &g, uin
   W const m1 = static_cast<W>(g.fact_m1), m2 = static_cast<W>(g.fact_m2);
   int const lx = g.fact_lx, ly = g.fact_ly, nq = g.fact_nq;
   int const up = g.nbits - lx; // row rotation: the bottom row moves to the top
+  // The candidate (q, s, b, a) has position k = (q * ly + b) * lx + a in the
+  // tables and code 2 k + s; the minimiser is kept as a code (-1: x itself)
+  // and the tables are read only on the rare ties and stabiliser hits, which
+  // the branches skip when no lane of the wave has one.
   W best = x;
-  int best_key = -1, best_e = -1, best_s = 0;
+  int best_code = -1;
   double nn = 0.0;
   for (int q = 0; q < nq; ++q) {
     W const y = group_apply<W, D>(g, g.fact_q[q], x);
-    size_t const base = static_cast<size_t>(q) * lx * ly;
 #pragma unroll
     for (int s = 0; s < (SI != 0 ? 2 : 1); ++s) {
-      double const sgn = (s == 1 && SI == -1) ? -1.0 : 1.0;
       W row = s == 1 ? static_cast<W>(y ^ flip) : y;
       for (int b = 0; b < ly; ++b) {
         W w = row;
+        int const k0 = (q * ly + b) * lx;
         for (int a = 0; a < lx; ++a) {
-          size_t const k = base + static_cast<size_t>(b) * lx + a;
-          int const e = g.fact_elem[k];
-          int const key = g.fact_key[k] + s;
-          if (w < best || (w == best && key < best_key)) {
+          int const code = 2 * (k0 + a) + s;
+          if (w < best) {
             best = w;
-            best_key = key;
-            best_e = e;
-            best_s = s;
+            best_code = code;
+          } else if (w == best && best_code >= 0 && w != x) {
+            // tie below x: the lower rank in the reference's reduction wins
+            int const rank_new = g.fact_key[k0 + a] + s;
+            int const rank_old = g.fact_key[best_code >> 1] + (best_code & 1);
+            if (rank_new < rank_old) best_code = code;
           }
-          if (w == x) nn = nn + sgn * g.eig_re[e];
+          if (w == x) {
+            double const re = g.eig_re[g.fact_elem[k0 + a]];
+            nn = nn + ((s == 1 && SI == -1) ? -re : re);
+          }
           w = static_cast<W>(((w >> 1) & m1) | ((w << (lx - 1)) & m2));
         }
         if (ly > 1) row = static_cast<W>((row >> lx) | ((row << up) & flip));
@@ -251,12 +258,13 @@ __device__ __forceinline__ StateInfo state_info_factored(GroupView const &g, uin
   }
   StateInfo out;
   out.rep = static_cast<uint64_t>(best);
-  if (best_e < 0) {
+  if (best_code < 0) {
     out.c_re = 1.0;
     out.c_im = 0.0;
   } else {
-    double const cre = g.eig_re[best_e], cim = g.eig_im[best_e];
-    bool const neg = SI == -1 && best_s == 1;
+    int const e = g.fact_elem[best_code >> 1];
+    double const cre = g.eig_re[e], cim = g.eig_im[e];
+    bool const neg = SI == -1 && (best_code & 1);
     out.c_re = neg ? -cre : cre;
     out.c_im = neg ? -cim : cim;
   }

@@ -48,6 +48,7 @@ struct ls_amd_sector_batch {
   std::vector<Graph> graphs;
   uint64_t graph_clock = 0;
   int hermitian = -1; // every sector's M Hermitian: -1 untested, 0 no, 1 yes
+  hipStream_t capture_stream = nullptr; // graph capture only
   static void release(Graph &g) {
     if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
     if (g.graph != nullptr) (void)hipGraphDestroy(g.graph);
@@ -56,6 +57,7 @@ struct ls_amd_sector_batch {
   }
   ~ls_amd_sector_batch() {
     for (Graph &g : graphs) release(g);
+    if (capture_stream != nullptr) (void)hipStreamDestroy(capture_stream);
   }
 };
 
@@ -172,18 +174,18 @@ template <typename F> bool dispatch_columns(int k, F &&f) {
 
 // form: 0 FP64 in / out, 1 FP32 in / out
 bool launch_product(ls_amd_sector_batch const &b, int form, int k, void const *x, void *y, int lanes, int blocks,
-                    int xcd) {
+                    int xcd, hipStream_t st) {
   Csr const &c = b.csr;
   int64_t const n = c.rows;
   if (n == 0) return true;
   return dispatch_columns(k, [&]<int K>() -> bool {
     return dispatch_lanes(lanes, [&]<int LPR>() -> bool {
       if (form == 0) {
-        sector_spmv_kernel<K, LPR, double, double><<<blocks, kThreads, 0, stream()>>>(
+        sector_spmv_kernel<K, LPR, double, double><<<blocks, kThreads, 0, st>>>(
             n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, c.diag64.ptr, static_cast<double const *>(x),
             static_cast<double *>(y), xcd);
       } else {
-        sector_spmv_kernel<K, LPR, float, float><<<blocks, kThreads, 0, stream()>>>(
+        sector_spmv_kernel<K, LPR, float, float><<<blocks, kThreads, 0, st>>>(
             n, c.row_ptr.ptr, c.cols.ptr, c.vals32.ptr, c.diag32.ptr, static_cast<float const *>(x),
             static_cast<float *>(y), xcd);
       }
@@ -219,10 +221,16 @@ bool graph_product(ls_amd_sector_batch &b, int form, int k, void const *x, void 
       break;
     }
   if (hit == nullptr) {
-    LSAMD_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-    bool const ok = launch_product(b, form, k, x, y, b.lanes, b.blocks, b.xcd_aware);
+    // Captured on the batch's own stream, which carries nothing else: work
+    // that other threads put on the engine's stream meanwhile runs normally
+    // instead of landing in the graph.  The graph itself is launched on the
+    // engine's stream, in order with the caller's other work.
+    if (b.capture_stream == nullptr)
+      LSAMD_HIP(hipStreamCreateWithFlags(&b.capture_stream, hipStreamNonBlocking));
+    LSAMD_HIP(hipStreamBeginCapture(b.capture_stream, hipStreamCaptureModeThreadLocal));
+    bool const ok = launch_product(b, form, k, x, y, b.lanes, b.blocks, b.xcd_aware, b.capture_stream);
     hipGraph_t gr = nullptr;
-    hipError_t const e = hipStreamEndCapture(stream(), &gr);
+    hipError_t const e = hipStreamEndCapture(b.capture_stream, &gr);
     if (!ok || e != hipSuccess) {
       if (gr != nullptr) (void)hipGraphDestroy(gr);
       if (!ok) return false;
@@ -369,10 +377,10 @@ bool sector_autotune(ls_amd_sector_batch &b, int k) {
         int64_t const tiles = (n + kThreads / lanes - 1) / (kThreads / lanes);
         int const blocks = per_cu > 0 ? std::max(kDies, (cu * per_cu / kDies) * kDies)
                                       : static_cast<int>(std::min<int64_t>((tiles + 7) / 8 * 8, kMaxGrid));
-        if (!launch_product(b, 1, k, x.ptr, y.ptr, lanes, blocks, xcd)) return false;
+        if (!launch_product(b, 1, k, x.ptr, y.ptr, lanes, blocks, xcd, stream())) return false;
         LSAMD_HIP(hipEventRecord(e0, stream()));
         for (int r = 0; r < 3; ++r)
-          if (!launch_product(b, 1, k, x.ptr, y.ptr, lanes, blocks, xcd)) return false;
+          if (!launch_product(b, 1, k, x.ptr, y.ptr, lanes, blocks, xcd, stream())) return false;
         LSAMD_HIP(hipEventRecord(e1, stream()));
         LSAMD_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -471,7 +479,7 @@ bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &opts, 
   if (bt.hermitian < 0) {
     // one complex column per state: the product's [re | im] rows interleave
     auto prod = [&](double const *x, double *y) -> bool {
-      if (!launch_product(bt, 0, 1, x, y, bt.lanes, bt.blocks, bt.xcd_aware)) return false;
+      if (!launch_product(bt, 0, 1, x, y, bt.lanes, bt.blocks, bt.xcd_aware, stream())) return false;
       LSAMD_HIP(hipGetLastError());
       return true;
     };
@@ -502,7 +510,7 @@ bool sector_eigensolve(ls_amd_sector_batch &bt, BlockKrylovOptions const &opts, 
     int const B = o.block;
     BlockProduct prod = [&](bool fp32, void const *x, void *y) -> bool {
       if (fp32) return graph_product(bt, 1, B, x, y);
-      if (!launch_product(bt, 0, B, x, y, bt.lanes, bt.blocks, bt.xcd_aware)) return false;
+      if (!launch_product(bt, 0, B, x, y, bt.lanes, bt.blocks, bt.xcd_aware, stream())) return false;
       LSAMD_HIP(hipGetLastError());
       return true;
     };

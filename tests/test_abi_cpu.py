@@ -90,6 +90,25 @@ int main(void) {
     assert [int(v) for v in out] == [72, 24, 32, 40, 64, 56, 16, 48, 56, 8, 48, 40, 32]
 
 
+def test_struct_layouts_match_the_reference_header_member_by_member():
+    """oracle/layout_probe.c compiled against include/lattice_symmetries_types.h
+    prints the same sizes and offsets of every member as the same probe
+    compiled against the reference's kernels/lattice_symmetries_types.h
+    (oracle/_ref/layout_ref, built by `make -C oracle` where the reference
+    tree is present)."""
+    ref = os.path.join(REPO, "oracle", "_ref", "layout_ref")
+    if not os.path.exists(ref):
+        pytest.skip("oracle/_ref/layout_ref not built (the reference tree is absent)")
+    want = subprocess.run([ref], check=True, capture_output=True, text=True).stdout
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "layout_ours")
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"),
+                        os.path.join(REPO, "oracle", "layout_probe.c"), "-o", exe], check=True)
+        got = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    assert got.splitlines() == want.splitlines()
+    assert len(want.splitlines()) > 50
+
+
 @pytest.mark.parametrize("n,gens,sectors", [
     (32, [models.chain_translation(32), models.chain_parity(32)], [0, 0]),
     (36, models.square_generators(6), [0, 0, 0, 0, 0]),
