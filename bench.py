@@ -14,6 +14,9 @@ restatement of the reference's CPU algorithm, on a bounded sample).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload chain-sectors   (56 sectors of the 28-site chain)
        python bench.py --workload configs         (16- and 32-site chains, 4x4 Hubbard)
+       python bench.py --workload gather-ceiling  (the FP64 kernel on synthetic gather windows)
+       python bench.py --workload order-model     (L2 misses of other orders of the 6x6 matrix)
+       python bench.py --workload copy-calibration (a copy of known size for the counters)
 """
 import argparse
 import json
@@ -446,6 +449,100 @@ def gather_ceiling(args):
     print(json.dumps({"workload": "gather-ceiling", "rows": rows, "per_row": per_row, "results": out}), flush=True)
 
 
+def order_model(args):
+    """Locality of the 6x6 product's gathers and the L2 misses an LRU model
+    of each die's L2 (4 MiB, 16 ways, 128 B lines) predicts for other orders
+    of the same matrix: the basis ordered by relabelled sites (rows and
+    columns renumbered), reverse Cuthill-McKee, and processing orders of the
+    rows (by smallest column, random)."""
+    import ctypes as C
+    import lsamd
+    from lsamd import models
+    pl = probes()
+    pl.lsamd_probe_l2_model.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.c_int, C.c_int,
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L = args.lattice
+    n, h, S, expr = workload(L)
+    b = lsamd.DeviceBasis(n, h, 1, S)
+    dim = b.build()
+    reps = b.states
+    op = lsamd.DeviceOperator(b, expr)
+    op.set_csr_packing(False)
+    nnz = op.build_csr(64)
+    rp, cols, _, _ = op.csr_arrays()
+    del op
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.uint32)
+    lens = np.diff(rp)
+    rows = np.repeat(np.arange(dim, dtype=np.int64), lens)
+    dist = np.abs(cols.astype(np.int64) - rows)
+    edges = [0, 16, 256, 4096, 65536, 1 << 20, 1 << 24]
+    hist = {f"[{a}, {b_})": float(c) / nnz for a, b_, c in zip(edges[:-1], edges[1:], np.histogram(dist, edges)[0])}
+    del dist
+
+    def misses(rp_, cols_):
+        m, a = C.c_int64(), C.c_int64()
+        if pl.lsamd_probe_l2_model(dim, rp_.ctypes.data, cols_.ctypes.data, 8, 4 << 20, 16, 128,
+                                   C.byref(m), C.byref(a)) != 0:
+            raise SystemExit(pl.lsamd_probe_error().decode())
+        return m.value
+
+    def reorder(pi, relabel_columns):
+        """Rows in the order pi (new row t is old row pi[t]); columns renumbered too when relabelled."""
+        rl = lens[pi]
+        rp2 = np.zeros(dim + 1, dtype=np.int64)
+        np.cumsum(rl, out=rp2[1:])
+        idx = np.repeat(rp[pi] - rp2[:-1], rl) + np.arange(nnz, dtype=np.int64)
+        c2 = cols[idx]
+        if relabel_columns:
+            inv = np.empty(dim, dtype=np.int64)
+            inv[pi] = np.arange(dim)
+            c2 = inv[c2]
+        return rp2, np.ascontiguousarray(c2, dtype=np.uint32)
+
+    def site_order(perm):
+        key = np.zeros(dim, dtype=np.uint64)
+        for i, p in enumerate(perm):
+            key |= ((reps >> np.uint64(p)) & np.uint64(1)) << np.uint64(i)
+        return np.argsort(key, kind="stable")
+
+    rc = lambda s_: divmod(s_, L)  # noqa: E731
+    orders = {
+        "stored (sorted representatives, row-major sites)": None,
+        "sites reversed": site_order(list(range(n))[::-1]),
+        "sites column-major": site_order([c * L + r for r in range(L) for c in range(L)]),
+        "sites snake": site_order([r * L + (c if r % 2 == 0 else L - 1 - c) for r in range(L) for c in range(L)]),
+        "sites checkerboard": site_order(sorted(range(n), key=lambda s_: (sum(rc(s_)) % 2, s_))),
+    }
+    rng = np.random.default_rng(1)
+    for q in range(2):
+        orders[f"sites random {q}"] = site_order(list(rng.permutation(n)))
+    out = {}
+    base = misses(rp, cols)
+    out["stored (sorted representatives, row-major sites)"] = base
+    log(f"stored order: {base / 1e6:.1f} M misses")
+    for name, pi in orders.items():
+        if pi is None:
+            continue
+        m = misses(*reorder(pi, True))
+        out[name] = m
+        log(f"{name}: {m / 1e6:.1f} M misses")
+    mincol = np.full(dim, np.iinfo(np.int64).max)
+    np.minimum.at(mincol, rows, cols.astype(np.int64))
+    out["rows by smallest column (processing order only)"] = misses(*reorder(np.argsort(mincol, kind="stable"), False))
+    out["rows random (processing order only)"] = misses(*reorder(rng.permutation(dim), False))
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    A = sp.csr_matrix((np.ones(nnz, dtype=np.int8), cols.astype(np.int32), rp), shape=(dim, dim))
+    out["reverse Cuthill-McKee (rows and columns)"] = misses(*reorder(reverse_cuthill_mckee(A, symmetric_mode=True),
+                                                                         True))
+    for k, v in out.items():
+        log(f"{k}: {v / 1e6:.1f} M misses ({v / base:.3f} of the stored order)")
+    print(json.dumps({"workload": "order-model", "states": dim, "nonzeros": nnz, "column_distance_fractions": hist,
+                      "model": "LRU per die: 8 dies, 4 MiB, 16 ways, 128 B lines; die d takes rows [n d/8, n (d+1)/8)",
+                      "x_line_misses": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,7 +558,7 @@ def main():
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--block-cycle", type=int, default=8)
     ap.add_argument("--workload", choices=["square6", "chain-sectors", "configs", "copy-calibration",
-                                           "gather-ceiling"],
+                                           "gather-ceiling", "order-model"],
                     default="square6")
     ap.add_argument("--copy-mib", type=int, default=2048)
     ap.add_argument("--chain", type=int, default=28)
@@ -494,6 +591,9 @@ def main():
         return
     if args.workload == "configs":
         configs_bench(args)
+        return
+    if args.workload == "order-model":
+        order_model(args)
         return
 
     n, h, S, expr = workload(args.lattice)

@@ -11,12 +11,18 @@
 //   one gather per nonzero.
 // lsamd_probe_copy: a plain device copy of `bytes` bytes, `reps` times, whose
 //   known traffic scales the profiler's FETCH_SIZE / WRITE_SIZE counters.
+// lsamd_probe_l2_model (host code): an LRU model of the L2 of each
+//   accelerator die (l2_bytes, `ways`-way, line_bytes lines) fed the x
+//   gathers of a CSR product in which die d processes rows
+//   [n d / dies, n (d + 1) / dies) in order; returns the misses and
+//   accesses of all dies.
 //
 // Both return 0, or -1 with a message in lsamd_probe_error().
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "spmv_kernels.hpp"
@@ -196,5 +202,60 @@ extern "C" __attribute__((visibility("default"))) int lsamd_probe_copy(int64_t b
   (void)hipEventDestroy(e1);
   PROBE_HIP(hipStreamSynchronize(s));
   *ms_out = static_cast<double>(ms) / reps;
+  return 0;
+}
+
+extern "C" __attribute__((visibility("default"))) int lsamd_probe_l2_model(int64_t n, int64_t const *rp,
+                                                                         uint32_t const *cols, int dies,
+                                                                         int64_t l2_bytes, int ways, int line_bytes,
+                                                                         int64_t *misses_out, int64_t *accesses_out) {
+  if (n < 0 || rp == nullptr || cols == nullptr || dies < 1 || dies > 64 || ways < 1 || line_bytes < 8 ||
+      (line_bytes & (line_bytes - 1)) != 0 || l2_bytes < static_cast<int64_t>(ways) * line_bytes)
+    return failed("lsamd_probe_l2_model: bad arguments");
+  int64_t const sets = l2_bytes / line_bytes / ways;
+  int shift = 0;
+  while ((8 << shift) < line_bytes) ++shift; // x entries are 8 B
+  std::vector<int64_t> misses(dies, 0), accesses(dies, 0);
+  std::vector<std::thread> pool;
+  for (int d = 0; d < dies; ++d)
+    pool.emplace_back([&, d]() {
+      std::vector<uint64_t> tag(static_cast<size_t>(sets) * ways, ~uint64_t{0}), age(static_cast<size_t>(sets) * ways, 0);
+      uint64_t clock = 0;
+      int64_t m = 0, a = 0;
+      for (int64_t r = n * d / dies; r < n * (d + 1) / dies; ++r)
+        for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+          uint64_t const line = static_cast<uint64_t>(cols[k]) >> shift;
+          // hashed set index, as a real cache avoids power-of-two aliasing
+          int64_t const set = static_cast<int64_t>(((line * 0x9e3779b97f4a7c15ull) >> 40) % static_cast<uint64_t>(sets));
+          uint64_t *t = tag.data() + set * ways, *g = age.data() + set * ways;
+          ++clock;
+          ++a;
+          int hit = -1, victim = 0;
+          for (int w = 0; w < ways; ++w) {
+            if (t[w] == line) {
+              hit = w;
+              break;
+            }
+            if (g[w] < g[victim]) victim = w;
+          }
+          if (hit >= 0) {
+            g[hit] = clock;
+          } else {
+            ++m;
+            t[victim] = line;
+            g[victim] = clock;
+          }
+        }
+      misses[d] = m;
+      accesses[d] = a;
+    });
+  for (auto &t : pool) t.join();
+  int64_t mm = 0, aa = 0;
+  for (int d = 0; d < dies; ++d) {
+    mm += misses[d];
+    aa += accesses[d];
+  }
+  if (misses_out != nullptr) *misses_out = mm;
+  if (accesses_out != nullptr) *accesses_out = aa;
   return 0;
 }
