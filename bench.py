@@ -297,6 +297,8 @@ def general_case(lib, basis, L, dim, steps, rng):
         "csr_build_seconds": time.perf_counter() - t0, **fp64_product(lib, op, dim, steps, x, y)}
     del op
     for k, v in out.items():
+        traffic, src = profiled_traffic(f"square{L}_general", v["kernel"])
+        v.update({"traffic": traffic, "traffic_source": src, **traffic_rate(traffic, v["kernel_ms"])})
         log(f"{k}: {v['kernel_ms']:.3f} ms, {v['frac']:.3f} of the roofline, packed {v['packed']}")
     return out
 
