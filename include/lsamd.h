@@ -362,6 +362,11 @@ int ls_amd_operator_is_hermitian(ls_amd_operator *op, int mode);
    rows) on the persistent grid, bit 2 (with bit 1) the die-aware mapping
    over one workgroup per tile instead. */
 int ls_amd_operator_schedule(ls_amd_operator const *op, int *out);
+/* Process-wide: whether basis builds may use the translation-factorised
+   filter for groups of torus translations times point elements with
+   characters +-1 (1, the default) or always test element by element (0).
+   Both give the same representatives. */
+int ls_amd_set_factored_enumeration(int enable);
 int ls_amd_operator_set_schedule(ls_amd_operator *op, int lanes_f64,
                                  int flags_f64, int lanes_f32, int flags_f32);
 

@@ -945,6 +945,11 @@ extern "C" int ls_amd_operator_schedule(ls_amd_operator const *op, int *out) {
   return 0;
 }
 
+extern "C" int ls_amd_set_factored_enumeration(int enable) {
+  g_factored_enumeration.store(enable != 0);
+  return 0;
+}
+
 extern "C" int ls_amd_operator_set_schedule(ls_amd_operator *op, int lanes_f64, int flags_f64, int lanes_f32,
                                             int flags_f32) {
   if (op == nullptr) return fail("null argument") ? 0 : -1;

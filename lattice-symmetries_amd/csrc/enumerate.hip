@@ -21,6 +21,7 @@
 // tail elements one by one), so sectors whose characters are not +-1 keep
 // exactly the states the restated reference keeps.
 #include <algorithm>
+#include <atomic>
 
 #include "orbit_device.hpp"
 
@@ -301,6 +302,150 @@ __global__ __launch_bounds__(256) void enum_count_kernel(EnumParams p, GroupView
   if (lane < p.lanes) counts[lane] = count;
 }
 
+// ---------------------------------------------------------------------------
+// Factorised filter for groups G = {t o q} of torus translations t and point
+// elements q with real characters +-1 (GroupView::fact_*): the flag "every
+// image >= x" does not depend on the order of the elements, and the
+// stabiliser sum of +-1 terms is exact in any order, so the accepted states
+// are those of the filter above.
+// ---------------------------------------------------------------------------
+
+template <typename W> struct TorusRotation {
+  W m1, m2, flip;
+  int lx, up;
+  // translation by one column: bit r lx + c of the result is bit
+  // r lx + (c + 1) mod lx of w
+  __device__ __forceinline__ W column(W w) const {
+    return static_cast<W>(((w >> 1) & m1) | ((w << (lx - 1)) & m2));
+  }
+  // translation by one row: bit i of the result is bit (i + lx) mod n of w
+  __device__ __forceinline__ W row(W w) const { return static_cast<W>((w >> lx) | ((w << up) & flip)); }
+};
+
+template <typename W> __device__ __forceinline__ TorusRotation<W> torus_rotation(GroupView const &g) {
+  return TorusRotation<W>{static_cast<W>(g.fact_m1), static_cast<W>(g.fact_m2), static_cast<W>(g.flip), g.fact_lx,
+                          g.nbits - g.fact_lx};
+}
+
+// Records the decision on a lane's current candidate in its acceptance bits.
+struct LaneBits {
+  uint32_t *bits;
+  int64_t pos = 0;
+  uint32_t word = 0, count = 0;
+  __device__ __forceinline__ void decide(bool accept, int64_t &remaining) {
+    if (accept) {
+      word |= 1u << (pos & 31);
+      ++count;
+    }
+    ++pos;
+    --remaining;
+    if ((pos & 31) == 0 || remaining == 0) {
+      bits[(pos - 1) >> 5] = word;
+      word = 0;
+    }
+  }
+};
+
+// Pass 0: a candidate survives when no translation maps it below itself
+// (t(x) >= x, and with spin inversion t(x) ^ flip >= x, the identity's
+// flipped image included).  Ranked candidates of a fixed Hamming weight,
+// stepped by Gosper's rule.  Two translations per loop iteration (column
+// rotations along a row, then the next row), selected without branches;
+// lanes move to their next candidate independently.
+template <typename W, int SI>
+__global__ __launch_bounds__(256) void enum_translation_kernel(EnumParams p, GroupView g, uint32_t *__restrict__ bits,
+                                                               uint32_t *__restrict__ counts) {
+  int64_t const lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const first = lane * p.per_lane;
+  int64_t remaining = 0;
+  if (lane < p.lanes && first < p.total) remaining = min(p.per_lane, p.total - first);
+  W x = remaining > 0 ? static_cast<W>(enum_first(p, first)) : W{0};
+  LaneBits out{bits + lane * words_per_lane(p.per_lane)};
+  TorusRotation<W> const rot = torus_rotation<W>(g);
+  W const flip = static_cast<W>(g.flip);
+  int const lx = g.fact_lx, ly = g.fact_ly;
+  W row = x, w = x; // w = t_(a, b)(x), row = t_(0, b)(x)
+  int a = 0, b = 0;
+  while (__any(remaining > 0)) {
+    if (remaining > 0) {
+      bool reject = false;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        bool const live = b < ly;
+        bool const low = (SI != 0 && static_cast<W>(w ^ flip) < x) || ((a | b) != 0 && w < x);
+        reject = reject || (live && low);
+        bool const along = a + 1 < lx;
+        W const wc = rot.column(w), rr = rot.row(row);
+        w = along ? wc : rr;
+        row = along ? row : rr;
+        a = along ? a + 1 : 0;
+        b = along ? b : b + 1;
+      }
+      if (reject || b >= ly) {
+        out.decide(!reject, remaining);
+        if (remaining > 0) x = static_cast<W>(gosper_next(static_cast<uint64_t>(x)));
+        row = w = x;
+        a = b = 0;
+      }
+    }
+  }
+  if (lane < p.lanes) counts[lane] = out.count;
+}
+
+// Pass 1 over the survivors of pass 0 (listed candidates): one point element
+// q per loop iteration, the identity last (pass 0 has tested its
+// translations): q(x) by its Benes network, then every translation of q(x)
+// and of its flipped image.  Rejected as soon as an image is below x;
+// accepted after the last q when the stabiliser sum exceeds norm_eps.
+template <typename W, int D, int SI>
+__global__ __launch_bounds__(256) void enum_factored_kernel(EnumParams p, GroupView g, int q_identity,
+                                                            uint32_t *__restrict__ bits,
+                                                            uint32_t *__restrict__ counts) {
+  int64_t const lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const first = lane * p.per_lane;
+  int64_t remaining = 0;
+  if (lane < p.lanes && first < p.total) remaining = min(p.per_lane, p.total - first);
+  W x = remaining > 0 ? static_cast<W>(enum_first(p, first)) : W{0};
+  LaneBits out{bits + lane * words_per_lane(p.per_lane)};
+  TorusRotation<W> const rot = torus_rotation<W>(g);
+  W const flip = static_cast<W>(g.flip);
+  int const lx = g.fact_lx, ly = g.fact_ly, nq = g.fact_nq;
+  int i = 0;
+  double nn = 0.0;
+  while (__any(remaining > 0)) {
+    if (remaining > 0) {
+      int const q = (q_identity + 1 + i) % nq;
+      W const y = group_apply<W, D>(g, g.fact_q[q], x);
+      bool lower = false;
+#pragma unroll
+      for (int s = 0; s < (SI != 0 ? 2 : 1); ++s) {
+        W row = s == 1 ? static_cast<W>(y ^ flip) : y;
+        for (int b = 0; b < ly; ++b) {
+          W w = row;
+          int const k0 = (q * ly + b) * lx;
+          for (int a = 0; a < lx; ++a) {
+            lower = lower || w < x;
+            if (w == x) {
+              double const re = g.eig_re[g.fact_elem[k0 + a]];
+              nn = nn + ((s == 1 && SI == -1) ? -re : re);
+            }
+            w = rot.column(w);
+          }
+          row = rot.row(row);
+        }
+      }
+      ++i;
+      if (lower || i == nq) {
+        out.decide(!lower && nn > g.norm_eps, remaining);
+        if (remaining > 0) x = static_cast<W>(enum_step(p, first, out.pos, static_cast<uint64_t>(x)));
+        i = 0;
+        nn = 0.0;
+      }
+    }
+  }
+  if (lane < p.lanes) counts[lane] = out.count;
+}
+
 // Replays the candidate ranges and writes accepted states (all candidates
 // when bits == nullptr) starting at offsets[lane] (lane * per_lane when
 // offsets == nullptr).
@@ -440,6 +585,8 @@ int64_t lanes_for(int64_t total, int64_t &per_lane, bool listed = false) {
 }
 } // namespace
 
+std::atomic<bool> g_factored_enumeration{true};
+
 bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t upper,
                       DeviceArray<uint64_t> &out, int64_t &count, int64_t &candidates) {
   count = 0;
@@ -511,14 +658,26 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
 
   GroupData const &g = *spec.group;
   GroupView const v = g.view();
+  // The factorised filter: fixed Hamming weight, a group of torus
+  // translations times point elements with characters +-1, not fermionic.
+  int q_identity = -1;
+  if (g.factored() && p.hamming >= 0 && g_factored_enumeration.load())
+    for (size_t k = 0; k < g.h_fact_q.size(); ++k) {
+      bool id = true;
+      for (int d = 0; d < g.depth && id; ++d) id = g.h_masks[static_cast<size_t>(g.h_fact_q[k]) * g.depth + d] == 0;
+      if (id) q_identity = static_cast<int>(k);
+    }
+  bool const use_factored = q_identity >= 0;
   std::vector<int> const h_order =
       group_rejection_order(g, p.hamming >= 0 ? p.hamming : (p.hamming == kSpinful ? p.up + p.down : -1),
                             g.number_bits);
   DeviceArray<int> d_order;
   if (!d_order.upload(h_order.data(), h_order.size())) return false;
   // One filter pass over candidates q (ranked or listed): bits, counts,
-  // scan, then the accepted ones written to `dst` in order.
-  auto filter = [&](EnumParams const &q, DeviceArray<uint64_t> &dst, uint64_t &kept) -> bool {
+  // scan, then the accepted ones written to `dst` in order.  kind 0: the
+  // element-by-element filter; 1: translations only (pass 0 of the
+  // factorised filter); 2: the factorised test (its pass 1).
+  auto filter = [&](EnumParams const &q, DeviceArray<uint64_t> &dst, uint64_t &kept, int kind = 0) -> bool {
     int const nblocks = grid_for(q.lanes, 256);
     DeviceArray<uint32_t> bits, counts;
     DeviceArray<uint64_t> offsets;
@@ -526,6 +685,18 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
     if (!counts.allocate(static_cast<size_t>(q.lanes))) return false;
     if (!offsets.allocate(static_cast<size_t>(q.lanes))) return false;
     bool ok = dispatch_group(g, [&]<typename W, int D, int SI>() -> bool {
+      if constexpr (SI != kFermionic) {
+        if (kind == 1) {
+          enum_translation_kernel<W, SI><<<nblocks, 256, 0, stream()>>>(q, v, bits.ptr, counts.ptr);
+          LSAMD_HIP(hipGetLastError());
+          return true;
+        }
+        if (kind == 2) {
+          enum_factored_kernel<W, D, SI><<<nblocks, 256, 0, stream()>>>(q, v, q_identity, bits.ptr, counts.ptr);
+          LSAMD_HIP(hipGetLastError());
+          return true;
+        }
+      }
       int const depth = D > 0 ? D : g.depth;
       size_t const smem = ((static_cast<size_t>(g.size) * depth * sizeof(W) + 7) / 8) * 8 +
                           static_cast<size_t>(g.size) * (sizeof(double) + sizeof(int));
@@ -549,7 +720,23 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
   };
   timer_begin(kTimerEnumerate);
   uint64_t total = 0;
-  if (g.size <= kSurvivorElements) {
+  if (use_factored) {
+    // Pass 0: translations only, over every candidate.  Pass 1: the
+    // factorised test of the survivors (about one candidate in lx ly).
+    DeviceArray<uint64_t> survivors;
+    uint64_t nsurv = 0;
+    if (!filter(p, survivors, nsurv, 1)) return false;
+    EnumParams q = p;
+    q.list = survivors.ptr;
+    q.rank0 = 0;
+    q.total = static_cast<int64_t>(nsurv);
+    q.lanes = lanes_for(q.total, q.per_lane, true);
+    if (nsurv > 0) {
+      if (!filter(q, out, total, 2)) return false;
+    } else {
+      out.reset();
+    }
+  } else if (g.size <= kSurvivorElements) {
     p.limit = g.size;
     p.survive_only = 0;
     if (!filter(p, out, total)) return false;

@@ -257,6 +257,9 @@ struct EnumerationSpec {
   int number_sites = 0;
   int number_up = -1, number_down = -1;
 };
+// Whether enumerate_states may use the factorised filter for groups with a
+// translation factorisation (on by default; the result is the same).
+extern std::atomic<bool> g_factored_enumeration;
 // Writes the ascending representatives into `out`; returns false on error.
 bool enumerate_states(EnumerationSpec const &spec, uint64_t lower,
                       uint64_t upper, DeviceArray<uint64_t> &out,

@@ -244,6 +244,7 @@ SIGNATURES = {
     "ls_amd_operator_csr_info": (C.c_int, [_P, _i64p]),
     "ls_amd_operator_schedule": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "ls_amd_operator_set_schedule": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ls_amd_set_factored_enumeration": (C.c_int, [C.c_int]),
     "ls_amd_last_kernel_ms": (C.c_double, [C.c_int]),
     "ls_amd_lanczos": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, _P,
                                  C.POINTER(ls_amd_lanczos_result)]),
