@@ -12,7 +12,9 @@
 
 namespace lsamd {
 
-template <typename W, int D, int SI>
+// FACT: through the translation factorisation of the group (state_info_factored,
+// bit for bit the same results) instead of element by element.
+template <typename W, int D, int SI, bool FACT>
 __global__ __launch_bounds__(256) void state_info_kernel(GroupView g, int64_t count,
                                                          uint64_t const *__restrict__ xs,
                                                          int64_t x_stride,
@@ -22,7 +24,11 @@ __global__ __launch_bounds__(256) void state_info_kernel(GroupView g, int64_t co
                                                          double *__restrict__ norms) {
   int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  StateInfo const s = state_info_one<W, D, SI>(g, xs[i * x_stride]);
+  StateInfo s;
+  if constexpr (FACT)
+    s = state_info_factored<W, D, SI>(g, xs[i * x_stride]);
+  else
+    s = state_info_one<W, D, SI>(g, xs[i * x_stride]);
   reps[i * rep_stride] = s.rep;
   chars[2 * i] = s.c_re;
   chars[2 * i + 1] = s.c_im;
@@ -49,7 +55,15 @@ bool launch_state_info(GroupData const &g, int64_t count, uint64_t const *d_x, i
   if (count <= 0) return true;
   GroupView const v = g.view();
   return dispatch_group(g, [&]<typename W, int D, int SI>() -> bool {
-    state_info_kernel<W, D, SI><<<grid_for(count, 256), 256, 0, stream()>>>(
+    if constexpr (SI != kFermionic) {
+      if (g.factored()) {
+        state_info_kernel<W, D, SI, true><<<grid_for(count, 256), 256, 0, stream()>>>(
+            v, count, d_x, x_stride, d_reps, rep_stride, d_chars, d_norms);
+        LSAMD_HIP(hipGetLastError());
+        return true;
+      }
+    }
+    state_info_kernel<W, D, SI, false><<<grid_for(count, 256), 256, 0, stream()>>>(
         v, count, d_x, x_stride, d_reps, rep_stride, d_chars, d_norms);
     LSAMD_HIP(hipGetLastError());
     return true;
