@@ -227,7 +227,7 @@ template <typename W, int D, int SI, int KIND>
 __device__ __forceinline__ void off_diag_entry(GroupView const &g, OpView const &op,
                                                TermsLds const &l, int k, uint64_t alpha,
                                                double n_alpha, double scale, uint64_t &beta,
-                                               double &c_re, double &c_im) {
+                                               double &c_re, double &c_im, RowTables const *rt) {
   double const factor = ((__popcll(alpha & l.s[k]) & 1) ? -1.0 : 1.0) * scale;
   double re = l.v[2 * k] * factor, im = l.v[2 * k + 1] * factor;
   beta = alpha ^ l.x[k];
@@ -240,7 +240,8 @@ __device__ __forceinline__ void off_diag_entry(GroupView const &g, OpView const 
     }
   } else if constexpr (KIND == kPermutations || KIND == kFactored) {
     // BatchedOperator.chpl:232-247
-    StateInfo const s = KIND == kFactored ? state_info_factored<W, D, SI>(g, beta) : state_info_one<W, D, SI>(g, beta);
+    StateInfo const s =
+        KIND == kFactored ? state_info_factored<W, D, SI>(g, beta, rt) : state_info_one<W, D, SI>(g, beta);
     double const t_re = (re * s.norm) / n_alpha, t_im = (im * s.norm) / n_alpha;
     re = s.c_re * t_re - s.c_im * t_im;
     im = s.c_re * t_im + s.c_im * t_re;
@@ -301,6 +302,8 @@ __global__ __launch_bounds__(256) void otf_push_kernel(GroupView g, OpView op, d
                                                        double *__restrict__ y, int *err, uint64_t *err_state) {
   extern __shared__ uint64_t smem[];
   TermsLds const l = stage_terms(op.off, smem);
+  __shared__ RowTables rt;
+  if constexpr (KIND == kFactored) row_tables_init(g, rt);
   int const T = op.off.count;
   int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool const valid = i < op.n;
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(256) void otf_push_kernel(GroupView g, OpView op, d
     if (k < T) {
       uint64_t beta;
       double c_re, c_im;
-      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, CPLX ? 1.0 : xr, beta, c_re, c_im);
+      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, CPLX ? 1.0 : xr, beta, c_re, c_im, &rt);
       bool const nz = CPLX ? (c_re != 0 || c_im != 0) : (c_re != 0);
       if (nz) {
         int64_t const j = op.identity_index ? static_cast<int64_t>(beta) : index_lookup(op.idx, beta);
@@ -439,6 +442,8 @@ __global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
                                                        int *err, uint64_t *err_state) {
   extern __shared__ uint64_t smem[];
   TermsLds const l = stage_terms(op.off, smem);
+  __shared__ RowTables rt;
+  if constexpr (KIND == kFactored) row_tables_init(g, rt);
   int const T = op.off.count;
   int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool const valid = i < op.n;
@@ -460,7 +465,7 @@ __global__ __launch_bounds__(256) void csr_fill_kernel(GroupView g, OpView op,
     if (k < T) {
       uint64_t beta;
       double c_re, c_im;
-      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, 1.0, beta, c_re, c_im);
+      off_diag_entry<W, D, SI, KIND>(g, op, l, k, alpha, n_alpha, 1.0, beta, c_re, c_im, &rt);
       uint32_t col = kDropped;
       double val_re = 0.0, val_im = 0.0;
       if (CPLX ? (c_re != 0 || c_im != 0) : (c_re != 0)) {

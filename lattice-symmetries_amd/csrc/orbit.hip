@@ -22,11 +22,13 @@ __global__ __launch_bounds__(256) void state_info_kernel(GroupView g, int64_t co
                                                          int64_t rep_stride,
                                                          double *__restrict__ chars,
                                                          double *__restrict__ norms) {
+  __shared__ RowTables rt;
+  if constexpr (FACT) row_tables_init(g, rt);
   int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count) return;
   StateInfo s;
   if constexpr (FACT)
-    s = state_info_factored<W, D, SI>(g, xs[i * x_stride]);
+    s = state_info_factored<W, D, SI>(g, xs[i * x_stride], &rt);
   else
     s = state_info_one<W, D, SI>(g, xs[i * x_stride]);
   reps[i * rep_stride] = s.rep;

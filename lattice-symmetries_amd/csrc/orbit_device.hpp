@@ -213,8 +213,62 @@ __device__ __forceinline__ StateInfo state_info_one(GroupView const &g,
 // among equal images the lowest rank wins, as the strict comparisons of
 // that reduction decide.  rep, character and norm equal state_info_one's
 // bit for bit.
+//
+// With row tables (rows of at most kRowTableBits sites), the translations of
+// q(x) are not all formed: the least image has the least top row, and the
+// top row of t_(a, b)(y) is row (b - 1) mod ly of y rotated by a, so the
+// least rotation of each row (a table lookup) gives the least top row and
+// the few (a, b) that reach it; only those images are formed and compared
+// (none when that row exceeds the top row of the current minimum).  Images
+// equal to x need a row in the rotation class of x's top row, found the
+// same way.  Every image equal to the minimum, or to x, is still visited,
+// so the result is the same.
+constexpr int kRowTableBits = 8;
+
+struct RowTables {
+  uint8_t minrot[1 << kRowTableBits]; // least rotation of a row value
+  uint8_t amin[1 << kRowTableBits];   // rotations a that reach it (bit a)
+  uint64_t lo[kRowTableBits], hi[kRowTableBits];
+};
+
+__device__ __forceinline__ bool row_tables_apply(GroupView const &g) {
+  return g.fact_nq > 0 && g.fact_lx >= 1 && g.fact_lx <= kRowTableBits && g.fact_ly <= kRowTableBits;
+}
+
+// Fills the tables (every thread of the workgroup must call it).
+__device__ inline void row_tables_init(GroupView const &g, RowTables &t) {
+  if (row_tables_apply(g)) {
+    int const lx = g.fact_lx;
+    uint32_t const rm = (1u << lx) - 1;
+    for (uint32_t v = threadIdx.x; v <= rm; v += blockDim.x) {
+      uint32_t best = 0xFFu, mask = 0;
+      for (int a = 0; a < lx; ++a) {
+        uint32_t const r = ((v >> a) | (v << (lx - a))) & rm; // rotation by a
+        if (r < best) {
+          best = r;
+          mask = 1u << a;
+        } else if (r == best) {
+          mask |= 1u << a;
+        }
+      }
+      t.minrot[v] = static_cast<uint8_t>(best);
+      t.amin[v] = static_cast<uint8_t>(mask);
+    }
+    if (static_cast<int>(threadIdx.x) < lx) {
+      int const a = static_cast<int>(threadIdx.x);
+      uint64_t lo = 0, hi = 0;
+      for (int r = 0; r < g.fact_ly; ++r)
+        for (int c = 0; c < lx; ++c) (c < lx - a ? lo : hi) |= uint64_t{1} << (r * lx + c);
+      t.lo[a] = lo;
+      t.hi[a] = hi;
+    }
+  }
+  __syncthreads();
+}
+
 template <typename W, int D, int SI>
-__device__ __forceinline__ StateInfo state_info_factored(GroupView const &g, uint64_t x64) {
+__device__ __forceinline__ StateInfo state_info_factored(GroupView const &g, uint64_t x64,
+                                                        RowTables const *rt = nullptr) {
   W const x = static_cast<W>(x64);
   W const flip = static_cast<W>(g.flip);
   W const m1 = static_cast<W>(g.fact_m1), m2 = static_cast<W>(g.fact_m2);
@@ -227,6 +281,73 @@ __device__ __forceinline__ StateInfo state_info_factored(GroupView const &g, uin
   W best = x;
   int best_code = -1;
   double nn = 0.0;
+  if (rt != nullptr && row_tables_apply(g)) {
+    uint32_t const rm = (1u << lx) - 1;
+    int const top = (ly - 1) * lx;
+    uint32_t const xt = static_cast<uint32_t>(x >> top) & rm;
+    uint32_t const xcls = rt->minrot[xt];
+    // t_(0, b)(y): bit i is bit (i + b lx) mod n of y; t_(a, 0): bit (r, c) is
+    // bit (r, (c + a) mod lx)
+    auto rows_by = [&](W y, int b) -> W {
+      return b == 0 ? y : static_cast<W>((y >> (b * lx)) | ((y << (g.nbits - b * lx)) & flip));
+    };
+    auto columns_by = [&](W y, int a) -> W {
+      return static_cast<W>(((y >> a) & static_cast<W>(rt->lo[a])) | ((y << (lx - a)) & static_cast<W>(rt->hi[a])));
+    };
+    for (int q = 0; q < nq; ++q) {
+      W const y = group_apply<W, D>(g, g.fact_q[q], x);
+#pragma unroll
+      for (int s = 0; s < (SI != 0 ? 2 : 1); ++s) {
+        W const ys = s == 1 ? static_cast<W>(y ^ flip) : y;
+        // least rotation of every row; rows reaching the least one (cmask)
+        // and rows in the rotation class of x's top row (smask)
+        uint32_t mr[kRowTableBits];
+        uint32_t m = 0xFFu, smask = 0, cmask = 0;
+#pragma unroll
+        for (int j = 0; j < kRowTableBits; ++j) {
+          mr[j] = j < ly ? rt->minrot[static_cast<uint32_t>(ys >> (j * lx)) & rm] : 0xFFu;
+          m = mr[j] < m ? mr[j] : m;
+          smask |= (j < ly && mr[j] == xcls ? 1u : 0u) << j;
+        }
+#pragma unroll
+        for (int j = 0; j < kRowTableBits; ++j) cmask |= (j < ly && mr[j] == m ? 1u : 0u) << j;
+        if (m > (static_cast<uint32_t>(best >> top) & rm)) cmask = 0;
+        for (; cmask != 0; cmask &= cmask - 1) {
+          int const j = __ffs(cmask) - 1;
+          uint32_t const vj = static_cast<uint32_t>(ys >> (j * lx)) & rm;
+          int const b = j + 1 == ly ? 0 : j + 1;
+          W const yb = rows_by(ys, b);
+          for (uint32_t am = rt->amin[vj]; am != 0; am &= am - 1) {
+            int const a = __ffs(am) - 1;
+            W const w = columns_by(yb, a);
+            int const k = (q * ly + b) * lx + a;
+            int const code = 2 * k + s;
+            if (w < best) {
+              best = w;
+              best_code = code;
+            } else if (w == best && best_code >= 0 && w != x) {
+              int const rank_new = g.fact_key[k] + s;
+              int const rank_old = g.fact_key[best_code >> 1] + (best_code & 1);
+              if (rank_new < rank_old) best_code = code;
+            }
+          }
+        }
+        for (; smask != 0; smask &= smask - 1) {
+          int const j = __ffs(smask) - 1;
+          uint32_t const vj = static_cast<uint32_t>(ys >> (j * lx)) & rm;
+          int const b = j + 1 == ly ? 0 : j + 1;
+          W const yb = rows_by(ys, b);
+          for (int a = 0; a < lx; ++a) {
+            if ((((vj >> a) | (vj << (lx - a))) & rm) != xt) continue;
+            if (columns_by(yb, a) == x) {
+              double const re = g.eig_re[g.fact_elem[(q * ly + b) * lx + a]];
+              nn = nn + ((s == 1 && SI == -1) ? -re : re);
+            }
+          }
+        }
+      }
+    }
+  } else {
   for (int q = 0; q < nq; ++q) {
     W const y = group_apply<W, D>(g, g.fact_q[q], x);
 #pragma unroll
@@ -255,6 +376,7 @@ __device__ __forceinline__ StateInfo state_info_factored(GroupView const &g, uin
         if (ly > 1) row = static_cast<W>((row >> lx) | ((row << up) & flip));
       }
     }
+  }
   }
   StateInfo out;
   out.rep = static_cast<uint64_t>(best);
