@@ -349,12 +349,22 @@ struct LaneBits {
 // Pass 0: a candidate survives when no translation maps it below itself
 // (t(x) >= x, and with spin inversion t(x) ^ flip >= x, the identity's
 // flipped image included).  Ranked candidates of a fixed Hamming weight,
-// stepped by Gosper's rule.  Two translations per loop iteration (column
-// rotations along a row, then the next row), selected without branches;
-// lanes move to their next candidate independently.
+// stepped by Gosper's rule.
+//
+// With row tables (RowTables, rows and columns of at most 8 sites) a
+// candidate is decided in one step: an image below x needs a top row at
+// most x's top row, and the top rows of the translations of x are the
+// rotations of its rows, so x survives only if its top row is the least
+// rotation of every row, and then only the translations that reach that
+// row are formed and compared (the same for x ^ flip).  Otherwise two
+// translations per loop iteration (column rotations along a row, then the
+// next row), selected without branches; lanes move to their next candidate
+// independently.
 template <typename W, int SI>
 __global__ __launch_bounds__(256) void enum_translation_kernel(EnumParams p, GroupView g, uint32_t *__restrict__ bits,
                                                                uint32_t *__restrict__ counts) {
+  __shared__ RowTables rt;
+  row_tables_init(g, rt);
   int64_t const lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t const first = lane * p.per_lane;
   int64_t remaining = 0;
@@ -364,6 +374,47 @@ __global__ __launch_bounds__(256) void enum_translation_kernel(EnumParams p, Gro
   TorusRotation<W> const rot = torus_rotation<W>(g);
   W const flip = static_cast<W>(g.flip);
   int const lx = g.fact_lx, ly = g.fact_ly;
+  if (row_tables_apply(g)) {
+    uint32_t const rm = (1u << lx) - 1;
+    int const top = (ly - 1) * lx;
+    // t_(0, b) and t_(a, 0) as in state_info_factored
+    auto rows_by = [&](W y, int b) -> W {
+      return b == 0 ? y : static_cast<W>((y >> (b * lx)) | ((y << (g.nbits - b * lx)) & flip));
+    };
+    auto columns_by = [&](W y, int a) -> W {
+      return static_cast<W>(((y >> a) & static_cast<W>(rt.lo[a])) | ((y << (lx - a)) & static_cast<W>(rt.hi[a])));
+    };
+    // true when a translation of y is below x (xt: x's top row)
+    auto below = [&](W y, uint32_t xt) -> bool {
+      uint32_t mr[kRowTableBits];
+      uint32_t m = 0xFFu, cmask = 0;
+#pragma unroll
+      for (int j = 0; j < kRowTableBits; ++j) {
+        mr[j] = j < ly ? rt.minrot[static_cast<uint32_t>(y >> (j * lx)) & rm] : 0xFFu;
+        m = mr[j] < m ? mr[j] : m;
+      }
+      if (m < xt) return true;
+      if (m > xt) return false;
+#pragma unroll
+      for (int j = 0; j < kRowTableBits; ++j) cmask |= (j < ly && mr[j] == m ? 1u : 0u) << j;
+      bool low = false;
+      for (; cmask != 0 && !low; cmask &= cmask - 1) {
+        int const j = __ffs(cmask) - 1;
+        uint32_t const vj = static_cast<uint32_t>(y >> (j * lx)) & rm;
+        W const yb = rows_by(y, j + 1 == ly ? 0 : j + 1);
+        for (uint32_t am = rt.amin[vj]; am != 0; am &= am - 1) low = low || columns_by(yb, __ffs(am) - 1) < x;
+      }
+      return low;
+    };
+    while (remaining > 0) {
+      uint32_t const xt = static_cast<uint32_t>(x >> top) & rm;
+      bool const reject = below(x, xt) || (SI != 0 && below(static_cast<W>(x ^ flip), xt));
+      out.decide(!reject, remaining);
+      if (remaining > 0) x = static_cast<W>(gosper_next(static_cast<uint64_t>(x)));
+    }
+    if (lane < p.lanes) counts[lane] = out.count;
+    return;
+  }
   W row = x, w = x; // w = t_(a, b)(x), row = t_(0, b)(x)
   int a = 0, b = 0;
   while (__any(remaining > 0)) {
@@ -401,6 +452,9 @@ template <typename W, int D, int SI>
 __global__ __launch_bounds__(256) void enum_factored_kernel(EnumParams p, GroupView g, int q_identity,
                                                             uint32_t *__restrict__ bits,
                                                             uint32_t *__restrict__ counts) {
+  __shared__ RowTables rt;
+  row_tables_init(g, rt);
+  bool const tables = row_tables_apply(g);
   int64_t const lane = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t const first = lane * p.per_lane;
   int64_t remaining = 0;
@@ -417,8 +471,47 @@ __global__ __launch_bounds__(256) void enum_factored_kernel(EnumParams p, GroupV
       int const q = (q_identity + 1 + i) % nq;
       W const y = group_apply<W, D>(g, g.fact_q[q], x);
       bool lower = false;
+      if (tables) {
+        // as in state_info_factored: only the translations whose top row can
+        // be below x's, or equal to it, are formed
+        uint32_t const rm = (1u << lx) - 1;
+        int const top = (ly - 1) * lx;
+        uint32_t const xt = static_cast<uint32_t>(x >> top) & rm;
+        uint32_t const xcls = rt.minrot[xt];
 #pragma unroll
-      for (int s = 0; s < (SI != 0 ? 2 : 1); ++s) {
+        for (int s = 0; s < (SI != 0 ? 2 : 1); ++s) {
+          W const ys = s == 1 ? static_cast<W>(y ^ flip) : y;
+          uint32_t m = 0xFFu, smask = 0;
+#pragma unroll
+          for (int j = 0; j < kRowTableBits; ++j) {
+            uint32_t const mr = j < ly ? rt.minrot[static_cast<uint32_t>(ys >> (j * lx)) & rm] : 0xFFu;
+            m = mr < m ? mr : m;
+            smask |= (j < ly && mr == xcls ? 1u : 0u) << j;
+          }
+          lower = lower || m < xt;
+          // images with top row xt: compared with x, and counted when equal
+          if (m <= xt) {
+            for (; smask != 0; smask &= smask - 1) {
+              int const j = __ffs(smask) - 1;
+              uint32_t const vj = static_cast<uint32_t>(ys >> (j * lx)) & rm;
+              int const b = j + 1 == ly ? 0 : j + 1;
+              W const yb = b == 0 ? ys : static_cast<W>((ys >> (b * lx)) | ((ys << (g.nbits - b * lx)) & flip));
+              for (int a = 0; a < lx; ++a) {
+                if ((((vj >> a) | (vj << (lx - a))) & rm) != xt) continue;
+                W const w = static_cast<W>(((yb >> a) & static_cast<W>(rt.lo[a])) |
+                                           ((yb << (lx - a)) & static_cast<W>(rt.hi[a])));
+                lower = lower || w < x;
+                if (w == x) {
+                  double const re = g.eig_re[g.fact_elem[(q * ly + b) * lx + a]];
+                  nn = nn + ((s == 1 && SI == -1) ? -re : re);
+                }
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < (tables ? 0 : (SI != 0 ? 2 : 1)); ++s) {
         W row = s == 1 ? static_cast<W>(y ^ flip) : y;
         for (int b = 0; b < ly; ++b) {
           W w = row;
