@@ -436,6 +436,16 @@ int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size,
                             uint64_t seed, double *evals, double *residuals,
                             double *d_ritz_out, ls_amd_eig_info *info);
 
+/* The same solve, returning all nev Ritz vectors: d_evecs (device) receives
+   nev rows of number_states values, row e the eigenvector of evals[e] (unit
+   norm), the [number of eigenvalues][dimension] layout of the eigenvectors
+   the reference's Diagonalize.chpl:246-255 writes. */
+int ls_amd_block_eigenvectors(ls_amd_operator *op, int mode, int block_size,
+                              int nev, int max_blocks, int keep_blocks,
+                              int max_cycles, double tol, double switch_tol,
+                              uint64_t seed, double *evals, double *residuals,
+                              double *d_evecs, ls_amd_eig_info *info);
+
 /* Host utility: eigen-decomposition of a dense symmetric n x n matrix A
    (row-major) by Householder tridiagonalisation and implicit QR with
    Wilkinson shifts, as the Krylov solvers use for their projected

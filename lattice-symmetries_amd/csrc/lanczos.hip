@@ -545,9 +545,12 @@ void fill_info(ls_amd_eig_info *info, BlockKrylovResult const &r) {
 }
 } // namespace
 
-extern "C" int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size, int nev, int max_blocks,
-                                       int keep_blocks, int max_cycles, double tol, double switch_tol, uint64_t seed,
-                                       double *evals, double *residuals, double *d_ritz_out, ls_amd_eig_info *info) {
+// The thick-restart block solve behind ls_amd_block_eigensolve and
+// ls_amd_block_eigenvectors: the first n_out Ritz vectors (ascending
+// eigenvalues) go to d_out, n_out rows of number_states values.
+static int block_eigensolve_into(ls_amd_operator *op, int mode, int block_size, int nev, int max_blocks,
+                                 int keep_blocks, int max_cycles, double tol, double switch_tol, uint64_t seed,
+                                 double *evals, double *residuals, double *d_out, int n_out, ls_amd_eig_info *info) {
   if (op == nullptr || evals == nullptr) return fail("null argument") ? 0 : -1;
   if (mode != 0 && mode != 1) return fail("mode must be 0 (FP64 block CSR) or 1 (FP32 then FP64)") ? 0 : -1;
   Csr const &c = op->core.csr;
@@ -567,17 +570,34 @@ extern "C" int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_
   o.fp32 = mode == 1;
   int64_t const n = op->core.number_states;
   DeviceArray<double> Yb;
-  if (d_ritz_out != nullptr && !Yb.allocate(static_cast<size_t>(n) * block_size)) return -1;
+  if (d_out != nullptr && !Yb.allocate(static_cast<size_t>(n) * block_size)) return -1;
   BlockKrylovResult r;
-  if (!operator_block_solve(op->core, o, r, d_ritz_out != nullptr ? Yb.ptr : nullptr)) return -1;
+  if (!operator_block_solve(op->core, o, r, d_out != nullptr ? Yb.ptr : nullptr)) return -1;
   std::copy(r.evals.begin(), r.evals.end(), evals);
   if (residuals != nullptr) std::copy(r.residuals.begin(), r.residuals.end(), residuals);
-  if (d_ritz_out != nullptr) {
-    if (!block_column(Yb.ptr, block_size, 0, d_ritz_out, n)) return -1;
+  if (d_out != nullptr) {
+    for (int e = 0; e < n_out; ++e)
+      if (!block_column(Yb.ptr, block_size, e, d_out + static_cast<size_t>(e) * n, n)) return -1;
     if (hipStreamSynchronize(stream()) != hipSuccess) return fail("stream synchronisation failed") ? 0 : -1;
   }
   fill_info(info, r);
   return 0;
+}
+
+extern "C" int ls_amd_block_eigensolve(ls_amd_operator *op, int mode, int block_size, int nev, int max_blocks,
+                                       int keep_blocks, int max_cycles, double tol, double switch_tol, uint64_t seed,
+                                       double *evals, double *residuals, double *d_ritz_out, ls_amd_eig_info *info) {
+  return block_eigensolve_into(op, mode, block_size, nev, max_blocks, keep_blocks, max_cycles, tol, switch_tol, seed,
+                               evals, residuals, d_ritz_out, 1, info);
+}
+
+extern "C" int ls_amd_block_eigenvectors(ls_amd_operator *op, int mode, int block_size, int nev, int max_blocks,
+                                         int keep_blocks, int max_cycles, double tol, double switch_tol,
+                                         uint64_t seed, double *evals, double *residuals, double *d_evecs,
+                                         ls_amd_eig_info *info) {
+  if (d_evecs == nullptr) return fail("null argument") ? 0 : -1;
+  return block_eigensolve_into(op, mode, block_size, nev, max_blocks, keep_blocks, max_cycles, tol, switch_tol, seed,
+                               evals, residuals, d_evecs, nev, info);
 }
 
 extern "C" int ls_amd_block_lanczos(ls_amd_operator *op, int mode, int block_size, int max_blocks, int k, double tol,

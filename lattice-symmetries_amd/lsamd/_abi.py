@@ -263,6 +263,9 @@ SIGNATURES = {
     "ls_amd_block_eigensolve": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                           C.c_double, C.c_uint64, _f64p, _f64p, _P,
                                           C.POINTER(ls_amd_eig_info)]),
+    "ls_amd_block_eigenvectors": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                            C.c_double, C.c_uint64, _f64p, _f64p, _P,
+                                            C.POINTER(ls_amd_eig_info)]),
     "ls_amd_sector_batch_create": (_P, [C.POINTER(_P), C.c_int]),
     "ls_amd_sector_batch_destroy": (None, [_P]),
     "ls_amd_sector_batch_count": (C.c_int, [_P]),
