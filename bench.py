@@ -272,6 +272,52 @@ def fp64_product(lib, op, dim, steps, x, y):
             "frac": achieved / HBM_PEAK_GBS}
 
 
+def hdf5_io(basis, x, dim):
+    """HDF5 interchange of the basis and one vector between device memory and
+    a file in the reference's layout (basis/representatives u64 [dim],
+    hamiltonian/eigenvectors f64 [1][dim]), through liblsamd_hdf5.so.  The
+    file is written to the temporary directory and read back at once, so the
+    reads come from the page cache; nothing is fsync'ed."""
+    import tempfile
+
+    import lsamd
+    from lsamd import hdf5 as H
+    if not H.available():
+        return {"available": False}
+    path = os.path.join(tempfile.gettempdir(), f"lsamd_bench_{os.getpid()}.h5")
+    out = {"available": True, "file": "basis/representatives u64 [dim] + hamiltonian/eigenvectors f64 [1][dim]"}
+    try:
+        H.make_file(path)
+        for g in ("basis", "hamiltonian"):
+            H.make_group(path, g)
+        t0 = time.perf_counter()
+        H.write_dataset_as_blocks(path, "basis/representatives", basis.states)
+        t_w = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        states = H.read_dataset_as_blocks(path, "basis/representatives", rank=1, dtype=np.uint64)
+        lsamd.DeviceArray.from_numpy(states)
+        t_r = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        H.write_device_dataset(path, "hamiltonian/eigenvectors", x, np.float64, (1, dim))
+        t_wv = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        dv, _ = H.read_device_dataset(path, "hamiltonian/eigenvectors", np.float64, 2)
+        t_rv = time.perf_counter() - t0
+        ok = bool(np.array_equal(dv.to_numpy(np.float64, (dim,)), x.to_numpy(np.float64, (dim,)))
+                  and np.array_equal(states, basis.states))
+        gb = 8 * dim / 1e9
+        out.update({"representatives_write_GBs": gb / t_w, "representatives_read_GBs": gb / t_r,
+                    "vector_write_GBs": gb / t_wv, "vector_read_GBs": gb / t_rv, "bytes_each": 8 * dim,
+                    "round_trip_exact": ok,
+                    "note": "device -> file and file -> device, PCIe copies included; reads from the page cache"})
+        log(f"HDF5: representatives {gb / t_w:.2f} GB/s write, {gb / t_r:.2f} GB/s read; vector "
+            f"{gb / t_wv:.2f} GB/s write, {gb / t_rv:.2f} GB/s read (round trip exact: {ok})")
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    return out
+
+
 def general_case(lib, basis, L, dim, steps, rng):
     """The FP64 product where the value table does not apply: the same 6x6
     matrix stored unpacked (column and FP64 value per entry), and a model
@@ -706,6 +752,9 @@ def main():
     lib.ls_amd_synchronize()
     t_otf = time.perf_counter() - t0
 
+    # ---- HDF5 interchange of the basis and a vector ----
+    h5 = hdf5_io(basis, x, dim)
+
     # ---- ground state: plain FP64 vs FP32 block of 8 + FP64 correction ----
     e0 = {}
     if not args.no_lanczos:
@@ -757,6 +806,7 @@ def main():
                         "gather_model_GBs": block_gather_gbs},
         "general_case": general,
         "otf_fp64": {"seconds": t_otf, "nonzeros_per_s": nnz / t_otf},
+        "hdf5_io": h5,
         "ground_state": e0,
     }
     if not args.no_cpu_baseline:
