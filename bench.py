@@ -497,6 +497,41 @@ def gather_ceiling(args):
     print(json.dumps({"workload": "gather-ceiling", "rows": rows, "per_row": per_row, "results": out}), flush=True)
 
 
+def column_distance(args):
+    """How far the 6x6 product's columns lie from their row: the distribution
+    of |column - row| over every stored entry, and the share of entries that
+    a window of W vector entries around a tile of T rows would hold (an
+    LDS-resident slice of x; 160 KB of LDS holds 20 480 FP64 values)."""
+    import lsamd
+    n, h, S, expr = workload(args.lattice)
+    basis = lsamd.DeviceBasis(n, h, 1, S)
+    dim = basis.build()
+    op = lsamd.DeviceOperator(basis, expr)
+    op.build_csr(64)
+    rp, cols, vals, diag = op.csr_arrays()
+    del vals, diag
+    hist = np.zeros(40, dtype=np.int64)
+    tiles, wins = (256, 4096), (4096, 20480, 65536, 131072)
+    inwin = {(T, W): 0 for T in tiles for W in wins if W > T}
+    for r0 in range(0, dim, 1 << 20):
+        r1 = min(dim, r0 + (1 << 20))
+        rows = np.repeat(np.arange(r0, r1, dtype=np.int64), np.diff(rp[r0:r1 + 1]))
+        c = cols[rp[r0]:rp[r1]].astype(np.int64)
+        hist += np.bincount(np.floor(np.log2(np.abs(c - rows) + 1)).astype(np.int64), minlength=40)[:40]
+        for (T, W) in inwin:
+            lo = (rows // T) * T - (W - T) // 2
+            inwin[(T, W)] += int(np.count_nonzero((c >= lo) & (c < lo + W)))
+        log(f"rows {r1} of {dim}")
+    nnz = int(cols.size)
+    cum = np.cumsum(hist) / nnz
+    print(json.dumps({"workload": f"column distances of the {args.lattice}x{args.lattice} J1-J2 product",
+                      "states": int(dim), "off_diagonal_nonzeros": nnz,
+                      "share_with_distance_below_2^k": {int(k): round(float(cum[k - 1]), 4)
+                                                        for k in range(1, 25)},
+                      "share_in_window": {f"tile {T} rows, window {W}": round(v / nnz, 4)
+                                          for (T, W), v in inwin.items()}}), flush=True)
+
+
 def order_model(args):
     """Locality of the 6x6 product's gathers and the L2 misses an LRU model
     of each die's L2 (4 MiB, 16 ways, 128 B lines) predicts for other orders
@@ -606,7 +641,7 @@ def main():
     ap.add_argument("--lanczos-steps", type=int, default=200)
     ap.add_argument("--block-cycle", type=int, default=8)
     ap.add_argument("--workload", choices=["square6", "chain-sectors", "configs", "copy-calibration",
-                                           "gather-ceiling", "order-model"],
+                                           "gather-ceiling", "order-model", "column-distance"],
                     default="square6")
     ap.add_argument("--copy-mib", type=int, default=2048)
     ap.add_argument("--chain", type=int, default=28)
@@ -642,6 +677,9 @@ def main():
         return
     if args.workload == "order-model":
         order_model(args)
+        return
+    if args.workload == "column-distance":
+        column_distance(args)
         return
 
     n, h, S, expr = workload(args.lattice)
