@@ -40,12 +40,7 @@ std::mutex g_table_mutex;
 // Host array -> device array.
 template <typename T> bool to_device(DeviceArray<T> &d, T const *h, size_t n) { return d.upload(h, n); }
 
-template <typename T> bool to_host(T *h, T const *d, size_t n) {
-  if (n == 0) return true;
-  LSAMD_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, stream()));
-  LSAMD_HIP(hipStreamSynchronize(stream()));
-  return true;
-}
+template <typename T> bool to_host(T *h, T const *d, size_t n) { return copy_to_host(h, d, n * sizeof(T)); }
 
 // Gathers a strided host array into a contiguous vector.
 std::vector<uint64_t> gather_strided(uint64_t const *p, ptrdiff_t n, ptrdiff_t stride) {
@@ -103,6 +98,10 @@ struct CachedOperator {
   bool use_csr = false;
   size_t device_bytes = 0;
   uint64_t last_use = 0;
+  // x and y of ls_chpl_matrix_vector_product, kept between calls; a call
+  // that finds them in use by another thread allocates its own
+  std::mutex vectors_busy;
+  DeviceArray<double> dx, dy;
 };
 std::map<ls_hs_operator const *, std::shared_ptr<CachedOperator>> g_op_cache;
 std::mutex g_op_mutex;
@@ -258,6 +257,7 @@ std::shared_ptr<CachedOperator> cached_operator(ls_hs_operator const *op) {
     c->device_bytes = csr_bytes(c->core.csr);
   }
   c->device_bytes += static_cast<size_t>(c->core.number_states) * 8 * (c->owned_reps.ptr != nullptr ? 2 : 1);
+  c->device_bytes += static_cast<size_t>(c->core.number_states) * 16; // x and y
   c->last_use = ++g_op_clock;
   g_op_cache[op] = c;
   enforce_cache_capacity(c.get());
@@ -612,15 +612,23 @@ extern "C" void ls_chpl_matrix_vector_product(ls_hs_operator *matrixPtr, int num
   bool const has_diag = core.diag.count > 0;
   bool const has_off = core.off.count > 0;
   if (!has_diag && !has_off) return; // DistributedMatrixVector.chpl:1054-1057
-  DeviceArray<double> x, y;
-  bool ok = to_device(x, xPtr, n) && (has_diag ? y.allocate(n) : to_device(y, yPtr, n));
+  // x (and y when the product accumulates into it) go through the pinned
+  // staging copies; the device vectors stay with the cached operator
+  std::unique_lock<std::mutex> vl(c->vectors_busy, std::try_to_lock);
+  DeviceArray<double> own_x, own_y;
+  DeviceArray<double> &x = vl.owns_lock() ? c->dx : own_x;
+  DeviceArray<double> &y = vl.owns_lock() ? c->dy : own_y;
+  size_t const bytes = sizeof(double) * static_cast<size_t>(n);
+  bool ok = (x.count == static_cast<size_t>(n) || x.allocate(n)) &&
+            (y.count == static_cast<size_t>(n) || y.allocate(n)) && copy_to_device(x.ptr, xPtr, bytes) &&
+            (has_diag || copy_to_device(y.ptr, yPtr, bytes));
   if (ok) {
     if (c->use_csr)
       ok = operator_csr_f64(core, x.ptr, y.ptr, !has_diag);
     else
       ok = operator_matvec_otf(core, x.ptr, y.ptr, !has_diag, false);
   }
-  ok = ok && to_host(yPtr, y.ptr, n);
+  ok = ok && copy_to_host(yPtr, y.ptr, bytes);
   if (!ok) raise_last();
 }
 

@@ -41,6 +41,13 @@ void set_last_error(std::string const &msg);
 bool ensure_device();
 hipStream_t stream();
 
+// Copies between pageable host memory and device memory through pinned
+// staging buffers (staging.cpp).  copy_to_device: on return the data is in
+// device memory, ordered before later work on stream().  copy_to_host: waits
+// for the work queued on stream(), returns when h holds the data.
+bool copy_to_device(void *d, void const *h, size_t bytes);
+bool copy_to_host(void *h, void const *d, size_t bytes);
+
 // Kernel timing by family (see ls_amd_last_kernel_ms).
 enum TimerFamily {
   kTimerCsrF64 = 0,
@@ -89,10 +96,7 @@ template <typename T> struct DeviceArray {
   }
   bool upload(T const *host, size_t n) {
     if (!allocate(n)) return false;
-    if (n == 0) return true;
-    LSAMD_HIP(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, stream()));
-    LSAMD_HIP(hipStreamSynchronize(stream()));
-    return true;
+    return copy_to_device(ptr, host, n * sizeof(T));
   }
   void reset() {
     if (ptr != nullptr) (void)hipFree(ptr);
