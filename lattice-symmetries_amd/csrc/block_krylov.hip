@@ -57,10 +57,12 @@ constexpr int kChunkRows = 2048;
 constexpr double kReorthDrop = 1e-6;
 
 // partial[(c * nblk + i) * B*B + p*B + q] = sum over rows r of chunk c of
-// A_i[r][p] * W[r][q], with A_i = A + i * a_stride.  Blocks are stored as TA /
-// TW (float or double); products and sums are FP64.
+// A_i[r][p] * W[r][q], with A_i = A + i * a_stride, except that the last
+// block is `extra` when given (A_{nblk-1} = extra).  Blocks are stored as TA
+// / TW (float or double); products and sums are FP64.
 template <int B, typename TA, typename TW>
 __global__ __launch_bounds__(kThreads) void seg_gram_kernel(TA const *__restrict__ A, int64_t a_stride, int nblk,
+                                                            TA const *__restrict__ extra,
                                                             TW const *__restrict__ W,
                                                             int64_t const *__restrict__ chunk_rows,
                                                             double *__restrict__ partial) {
@@ -70,7 +72,8 @@ __global__ __launch_bounds__(kThreads) void seg_gram_kernel(TA const *__restrict
   int64_t const r0 = chunk_rows[2 * c], r1 = chunk_rows[2 * c + 1];
   int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = 0; i < nblk; ++i) {
-    TA const *__restrict__ Ai = A + static_cast<int64_t>(i) * a_stride;
+    TA const *__restrict__ Ai =
+        (extra != nullptr && i == nblk - 1) ? extra : A + static_cast<int64_t>(i) * a_stride;
     double acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0;
@@ -87,7 +90,10 @@ __global__ __launch_bounds__(kThreads) void seg_gram_kernel(TA const *__restrict
         for (int q = 0; q < B; ++q) acc[p * B + q] = fma(a[p], w[q], acc[p * B + q]);
     }
     // Transpose-reduce across the wave: after the steps below lane L holds
-    // the wave's sum for value index L & (V - 1).
+    // the wave's sum for value index L & (V - 1); then the waves' sums are
+    // added in wave order.  (Written out here and in seg_combine_kernel: a
+    // helper taking the accumulator array by reference made the compiler
+    // keep it in scratch memory.)
 #pragma unroll
     for (int h = V / 2; h > 0; h >>= 1) {
       bool const upper = (lane & h) != 0;
@@ -139,21 +145,29 @@ __global__ __launch_bounds__(kReduceThreads) void seg_reduce_kernel(double const
 
 // out[r][q] = beta * out[r][q] + sum_{i < nblk} sum_p A_i[r][p] C_s[i][p][q]
 // for the segment s of row r, summed in FP64 and stored as TO.  Each thread
-// owns whole rows, so out may alias one of the A_i (when TO is TA).
-template <int B, typename TO, typename TA>
+// owns whole rows, so out may alias one of the A_i (when TO is TA).  With
+// GRAM, partial[c * B*B + p*B + q] also receives the Gram sum out^T out of
+// the chunk's stored rows, with the rows, order and reduction of
+// seg_gram_kernel, so it equals that kernel's result bit for bit.
+template <int B, typename TO, typename TA, bool GRAM>
 __global__ __launch_bounds__(kThreads) void seg_combine_kernel(TO *out, double beta, TA const *A,
                                                                int64_t a_stride, int nblk,
                                                                double const *__restrict__ coef,
                                                                int64_t const *__restrict__ chunk_rows,
-                                                               int const *__restrict__ chunk_seg) {
+                                                               int const *__restrict__ chunk_seg,
+                                                               double *__restrict__ partial) {
   constexpr int V = B * B;
   extern __shared__ double sc[];
+  __shared__ double sg[GRAM ? kThreads / 64 : 1][64];
   int const c = blockIdx.x;
   int const s = chunk_seg[c];
   int64_t const r0 = chunk_rows[2 * c], r1 = chunk_rows[2 * c + 1];
   double const *__restrict__ cs = coef + static_cast<int64_t>(s) * nblk * V;
   for (int t = threadIdx.x; t < nblk * V; t += kThreads) sc[t] = cs[t];
   __syncthreads();
+  double g[GRAM ? V : 1];
+#pragma unroll
+  for (int v = 0; v < (GRAM ? V : 1); ++v) g[v] = 0.0;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kThreads) {
     double acc[B];
 #pragma unroll
@@ -173,6 +187,38 @@ __global__ __launch_bounds__(kThreads) void seg_combine_kernel(TO *out, double b
     }
 #pragma unroll
     for (int q = 0; q < B; ++q) out[r * B + q] = static_cast<TO>(acc[q]);
+    if constexpr (GRAM) {
+      double o[B];
+#pragma unroll
+      for (int q = 0; q < B; ++q) o[q] = static_cast<double>(static_cast<TO>(acc[q]));
+#pragma unroll
+      for (int p = 0; p < B; ++p)
+#pragma unroll
+        for (int q = 0; q < B; ++q) g[p * B + q] = fma(o[p], o[q], g[p * B + q]);
+    }
+  }
+  if constexpr (GRAM) { // the reduction of seg_gram_kernel
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = V / 2; h > 0; h >>= 1) {
+      bool const upper = (lane & h) != 0;
+#pragma unroll
+      for (int v = 0; v < h; ++v) {
+        double const send = upper ? g[v] : g[v + h];
+        double const keep = upper ? g[v + h] : g[v];
+        g[v] = keep + __shfl_xor(send, h, 64);
+      }
+    }
+    double tot = g[0];
+    for (int off = V; off < 64; off <<= 1) tot += __shfl_xor(tot, off, 64);
+    sg[wave][lane] = tot;
+    __syncthreads();
+    if (threadIdx.x < V) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; ++w) sum += sg[w][threadIdx.x];
+      partial[static_cast<int64_t>(c) * V + threadIdx.x] = sum;
+    }
   }
 }
 
@@ -394,18 +440,29 @@ struct BlockAlgebra {
            d_out.allocate(static_cast<size_t>(seg.nseg) * nblk_max * V) &&
            d_coef.allocate(static_cast<size_t>(seg.nseg) * nblk_max * V);
   }
-  // out_host[(s * nblk + i) * V + p * B + q] = A_i^T W over segment s
+  // out_host[(s * nblk + i) * V + p * B + q] = A_i^T W over segment s; with
+  // `extra`, one more block follows the nblk strided ones: out_host then
+  // holds nblk + 1 matrices per segment, the last extra^T W.
   template <typename TA, typename TW>
-  bool gram(TA const *A, int64_t a_stride, int nblk, TW const *W, std::vector<double> &out_host) {
-    if (nblk > max_nblk) return fail("internal: too many blocks in a Gram product");
+  bool gram(TA const *A, int64_t a_stride, int nblk, TW const *W, std::vector<double> &out_host,
+            TA const *extra = nullptr) {
+    int const total = nblk + (extra != nullptr ? 1 : 0);
+    if (total > max_nblk) return fail("internal: too many blocks in a Gram product");
     int const V = B * B;
     if (seg.chunks > 0) {
       bool ok = dispatch_block(B, [&]<int BB>() -> bool {
-        seg_gram_kernel<BB, TA, TW><<<seg.chunks, kThreads, 0, stream()>>>(A, a_stride, nblk, W,
+        seg_gram_kernel<BB, TA, TW><<<seg.chunks, kThreads, 0, stream()>>>(A, a_stride, total, extra, W,
                                                                            seg.d_chunk_rows.ptr, partial.ptr);
         return true;
       });
       if (!ok) return false;
+    }
+    return reduce_to_host(total, out_host);
+  }
+  // Per-segment sums of the chunk partials of `nblk` Gram matrices, to the host.
+  bool reduce_to_host(int nblk, std::vector<double> &out_host) {
+    int const V = B * B;
+    if (seg.chunks > 0) {
       seg_reduce_kernel<<<dim3(seg.nseg, nblk), kReduceThreads, 0, stream()>>>(partial.ptr, nblk, V,
                                                                                 seg.d_seg_chunk.ptr, d_out.ptr);
       LSAMD_HIP(hipGetLastError());
@@ -428,23 +485,34 @@ struct BlockAlgebra {
     LSAMD_HIP(hipStreamSynchronize(stream()));
     return true;
   }
-  // out = beta * out + A * C (per segment), coefficients [s][i][p][q] on the host
+  // out = beta * out + A * C (per segment), coefficients [s][i][p][q] on the
+  // host.  With gram_out, also out^T out per segment (as gram(out, 1, out)
+  // would give it, bit for bit), from the same pass.
   template <typename TO, typename TA>
-  bool combine(TO *out, double beta, TA const *A, int64_t a_stride, int nblk, std::vector<double> const &coef_host) {
+  bool combine(TO *out, double beta, TA const *A, int64_t a_stride, int nblk, std::vector<double> const &coef_host,
+               std::vector<double> *gram_out = nullptr) {
     if (nblk > max_nblk) return fail("internal: too many blocks in a block update");
     int const V = B * B;
-    if (seg.chunks == 0) return true;
+    if (seg.chunks == 0) {
+      if (gram_out != nullptr) gram_out->assign(static_cast<size_t>(seg.nseg) * V, 0.0);
+      return true;
+    }
     h_coef = coef_host;
     LSAMD_HIP(hipMemcpyAsync(d_coef.ptr, h_coef.data(), sizeof(double) * h_coef.size(), hipMemcpyHostToDevice,
                              stream()));
     size_t const smem = sizeof(double) * static_cast<size_t>(nblk) * V;
     bool ok = dispatch_block(B, [&]<int BB>() -> bool {
-      seg_combine_kernel<BB, TO, TA><<<seg.chunks, kThreads, smem, stream()>>>(
-          out, beta, A, a_stride, nblk, d_coef.ptr, seg.d_chunk_rows.ptr, seg.d_chunk_seg.ptr);
+      if (gram_out != nullptr)
+        seg_combine_kernel<BB, TO, TA, true><<<seg.chunks, kThreads, smem, stream()>>>(
+            out, beta, A, a_stride, nblk, d_coef.ptr, seg.d_chunk_rows.ptr, seg.d_chunk_seg.ptr, partial.ptr);
+      else
+        seg_combine_kernel<BB, TO, TA, false><<<seg.chunks, kThreads, smem, stream()>>>(
+            out, beta, A, a_stride, nblk, d_coef.ptr, seg.d_chunk_rows.ptr, seg.d_chunk_seg.ptr, nullptr);
       return true;
     });
     if (!ok) return false;
     LSAMD_HIP(hipGetLastError());
+    if (gram_out != nullptr) return reduce_to_host(1, *gram_out); // synchronises
     // the host coefficients must outlive the copy
     LSAMD_HIP(hipStreamSynchronize(stream()));
     return true;
@@ -511,7 +579,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
     return fail("a segment is too small for (max_blocks + 1) * block_size Krylov vectors (" +
                 std::to_string(min_rows) + " rows)");
   BlockAlgebra alg;
-  if (!alg.init(seg_rows, listed, Bw, m)) return false;
+  if (!alg.init(seg_rows, listed, Bw, m + 1)) return false; // m blocks + W in one Gram pass
   int64_t const rows = alg.seg.rows;
   int64_t const blk = rows * Bw; // reals per block
   // FP64 buffers; while the product is FP32 the blocks are stored as FP32 in
@@ -575,8 +643,13 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
       std::vector<int> singular(ns, 0);
       int first = -1;
       T *src = X;
+      std::vector<double> Gnext; // the Gram of the first pass's output, from its update
       for (int pass = 0; pass < 2; ++pass) {
-        if (!alg.gram(src, blk, 1, src, G)) return false;
+        if (pass == 1) {
+          G.swap(Gnext);
+        } else if (!alg.gram(src, blk, 1, src, G)) {
+          return false;
+        }
         gram_to_field(G, static_cast<size_t>(ns), Bf, Gf);
         for (int s = 0; s < ns; ++s) {
           std::vector<F> Gs(Gf.begin() + static_cast<int64_t>(s) * Vf, Gf.begin() + static_cast<int64_t>(s + 1) * Vf);
@@ -599,7 +672,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         }
         if (first >= 0) break; // X is untouched: repair below and start over
         set_coef(Minv, 1);
-        if (!alg.combine(out, 0.0, src, blk, 1, coef)) return false;
+        if (!alg.combine(out, 0.0, src, blk, 1, coef, pass == 0 ? &Gnext : nullptr)) return false;
         src = out;
       }
       if (first < 0) {
@@ -672,7 +745,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
   int cycle = 0, cycles_run = 0, fp64_cycles = 0, blocks_done = 0;
   double prev_worst = INFINITY;
   bool done = false;
-  std::vector<double> C, Gw;
+  std::vector<double> C, Cw, Gw;
   std::vector<F> Cf, Csum, Mf, Gwf;
   // coefficients of Ritz block t: M[s][i] = Z_s[i-block rows, t-block columns]
   auto ritz_coef = [&](int t) {
@@ -705,11 +778,24 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         // column (squared norm left below kReorthDrop of ||W||^2, by
         // Pythagoras from ||W||^2 and V^H W), where one pass leaves mostly
         // rounding error (an exhausted Krylov space, a converged direction).
+        // Pass 0 also forms W^H W (the last block of its Gram pass).
         bool second = false;
-        if (!alg.gram(Wt, blk, 1, Wt, Gw)) return -1;
-        gram_to_field(Gw, static_cast<size_t>(ns), Bf, Gwf);
         for (int pass = 0; pass < (second ? 2 : 1); ++pass) {
-          if (!alg.gram(Vbase, blk, nb, Wt, C)) return -1;
+          if (pass == 0) {
+            if (!alg.gram(Vbase, blk, nb, Wt, Cw, Wt)) return -1;
+            C.resize(static_cast<size_t>(ns) * nb * Vw);
+            Gw.resize(static_cast<size_t>(ns) * Vw);
+            for (int s = 0; s < ns; ++s) {
+              auto const src = Cw.begin() + static_cast<std::ptrdiff_t>(s) * (nb + 1) * Vw;
+              std::copy(src, src + static_cast<std::ptrdiff_t>(nb) * Vw,
+                        C.begin() + static_cast<std::ptrdiff_t>(s) * nb * Vw);
+              std::copy(src + static_cast<std::ptrdiff_t>(nb) * Vw, src + static_cast<std::ptrdiff_t>(nb + 1) * Vw,
+                        Gw.begin() + static_cast<std::ptrdiff_t>(s) * Vw);
+            }
+            gram_to_field(Gw, static_cast<size_t>(ns), Bf, Gwf);
+          } else if (!alg.gram(Vbase, blk, nb, Wt, C)) {
+            return -1;
+          }
           gram_to_field(C, static_cast<size_t>(ns) * nb, Bf, Cf);
           if (pass == 0) {
             for (int s = 0; s < ns && !second; ++s)
@@ -789,7 +875,10 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         ++cycle;
         return to_fp64 ? 1 : 0;
       }
-      // Restart: Y_t = sum_i V_i Z_s[i-block, t-block], then the residual block
+      // Restart: Y_t = sum_i V_i Z_s[i-block, t-block], then the residual block.
+      // (One pass per Ritz block: a pass forming several at once has B*B
+      // FMAs per stored row and output and is bound by instruction issue,
+      // not memory, for B = 8.)
       int const r = o.keep;
       for (int t = 0; t < r; ++t) {
         ritz_coef(t);
@@ -841,8 +930,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
   std::vector<F> negTh(Th.size());
   for (size_t t = 0; t < Th.size(); ++t) negTh[t] = -Th[t];
   set_coef(negTh, 1);
-  if (!alg.combine(W.ptr, 1.0, Y.ptr, blk, 1, coef)) return false; // A Y - Y (Y^H A Y)
-  if (!alg.gram(W.ptr, blk, 1, W.ptr, Gr)) return false;
+  if (!alg.combine(W.ptr, 1.0, Y.ptr, blk, 1, coef, &Gr)) return false; // A Y - Y (Y^H A Y), and its Gram
   gram_to_field(Gr, static_cast<size_t>(ns), Bf, Grf);
   res.evals.assign(static_cast<size_t>(nseg_all) * o.nev, NAN);
   res.residuals.assign(static_cast<size_t>(nseg_all) * o.nev, NAN);
