@@ -154,6 +154,8 @@ void ls_hs_state_info_halide_kernel(ptrdiff_t batch_size,
 /* chapel/src/library.c:19-34 and LatticeSymmetries.chpl:17-32: initialise
    the device and register the four functions below in the global table. */
 void ls_chpl_init(void);
+/* Frees the engine's cached operators (those built for
+   ls_chpl_matrix_vector_product) and the scratch kept between basis builds. */
 void ls_chpl_finalize(void);
 /* StatesEnumeration.chpl:692-709.  Representatives of `p` in [lower, upper],
    ascending; `dest` receives a malloc'ed array and its freer. */

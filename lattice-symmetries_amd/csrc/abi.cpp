@@ -640,8 +640,11 @@ extern "C" void ls_chpl_init(void) {
 }
 
 extern "C" void ls_chpl_finalize(void) {
-  std::lock_guard<std::mutex> lock(g_op_mutex);
-  g_op_cache.clear();
+  {
+    std::lock_guard<std::mutex> lock(g_op_mutex);
+    g_op_cache.clear();
+  }
+  release_enumeration_scratch();
 }
 
 extern "C" void ls_amd_forget_operator(ls_hs_operator const *op) {

@@ -680,6 +680,31 @@ int64_t lanes_for(int64_t total, int64_t &per_lane, bool listed = false) {
 
 std::atomic<bool> g_factored_enumeration{true};
 
+namespace {
+// Scratch of the filter passes, kept between builds (grow-only) so that a
+// build makes no hipMalloc / hipFree calls of its own once the buffers are
+// large enough: a 6x6 build needs about 0.6 GB of candidate bits and 1 GB
+// of first-pass survivors.  Released by release_enumeration_scratch
+// (ls_chpl_finalize).  A build that finds them in use by another thread
+// allocates its own.
+struct EnumScratch {
+  std::mutex busy;
+  DeviceArray<uint32_t> bits, counts;
+  DeviceArray<uint64_t> offsets, survivors;
+};
+EnumScratch g_enum_scratch;
+
+template <typename T> bool at_least(DeviceArray<T> &a, size_t n) { return a.count >= n || a.allocate(n); }
+} // namespace
+
+void release_enumeration_scratch() {
+  std::lock_guard<std::mutex> lock(g_enum_scratch.busy);
+  g_enum_scratch.bits.reset();
+  g_enum_scratch.counts.reset();
+  g_enum_scratch.offsets.reset();
+  g_enum_scratch.survivors.reset();
+}
+
 bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t upper,
                       DeviceArray<uint64_t> &out, int64_t &count, int64_t &candidates) {
   count = 0;
@@ -761,22 +786,37 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
       if (id) q_identity = static_cast<int>(k);
     }
   bool const use_factored = q_identity >= 0;
-  std::vector<int> const h_order =
-      group_rejection_order(g, p.hamming >= 0 ? p.hamming : (p.hamming == kSpinful ? p.up + p.down : -1),
-                            g.number_bits);
+  // The element order of the element-by-element filter (kind 0), formed on
+  // first use: the factorised filter does not read it.
   DeviceArray<int> d_order;
-  if (!d_order.upload(h_order.data(), h_order.size())) return false;
+  auto order = [&]() -> int const * {
+    if (d_order.ptr == nullptr) {
+      std::vector<int> const h_order =
+          group_rejection_order(g, p.hamming >= 0 ? p.hamming : (p.hamming == kSpinful ? p.up + p.down : -1),
+                                g.number_bits);
+      if (!d_order.upload(h_order.data(), h_order.size())) return nullptr;
+    }
+    return d_order.ptr;
+  };
+  std::unique_lock<std::mutex> scratch_lock(g_enum_scratch.busy, std::try_to_lock);
+  EnumScratch own;
+  EnumScratch &sc = scratch_lock.owns_lock() ? g_enum_scratch : own;
   // One filter pass over candidates q (ranked or listed): bits, counts,
   // scan, then the accepted ones written to `dst` in order.  kind 0: the
   // element-by-element filter; 1: translations only (pass 0 of the
   // factorised filter); 2: the factorised test (its pass 1).
-  auto filter = [&](EnumParams const &q, DeviceArray<uint64_t> &dst, uint64_t &kept, int kind = 0) -> bool {
+  // `dst` receives exactly `kept` states (the scratch survivor list may be
+  // longer than its contents).
+  auto filter = [&](EnumParams const &q, DeviceArray<uint64_t> &dst, uint64_t &kept, int kind = 0,
+                    bool dst_is_scratch = false) -> bool {
     int const nblocks = grid_for(q.lanes, 256);
-    DeviceArray<uint32_t> bits, counts;
-    DeviceArray<uint64_t> offsets;
-    if (!bits.allocate(static_cast<size_t>(q.lanes * words_per_lane(q.per_lane)))) return false;
-    if (!counts.allocate(static_cast<size_t>(q.lanes))) return false;
-    if (!offsets.allocate(static_cast<size_t>(q.lanes))) return false;
+    DeviceArray<uint32_t> &bits = sc.bits, &counts = sc.counts;
+    DeviceArray<uint64_t> &offsets = sc.offsets;
+    if (!at_least(bits, static_cast<size_t>(q.lanes * words_per_lane(q.per_lane)))) return false;
+    if (!at_least(counts, static_cast<size_t>(q.lanes))) return false;
+    if (!at_least(offsets, static_cast<size_t>(q.lanes))) return false;
+    int const *d_ord = kind == 0 ? order() : nullptr;
+    if (kind == 0 && d_ord == nullptr) return false;
     bool ok = dispatch_group(g, [&]<typename W, int D, int SI>() -> bool {
       if constexpr (SI != kFermionic) {
         if (kind == 1) {
@@ -794,9 +834,9 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
       size_t const smem = ((static_cast<size_t>(g.size) * depth * sizeof(W) + 7) / 8) * 8 +
                           static_cast<size_t>(g.size) * (sizeof(double) + sizeof(int));
       if (smem <= 64 * 1024) {
-        enum_count_kernel<W, D, SI, true><<<nblocks, 256, smem, stream()>>>(q, v, d_order.ptr, bits.ptr, counts.ptr);
+        enum_count_kernel<W, D, SI, true><<<nblocks, 256, smem, stream()>>>(q, v, d_ord, bits.ptr, counts.ptr);
       } else {
-        enum_count_kernel<W, D, SI, false><<<nblocks, 256, 0, stream()>>>(q, v, d_order.ptr, bits.ptr, counts.ptr);
+        enum_count_kernel<W, D, SI, false><<<nblocks, 256, 0, stream()>>>(q, v, d_ord, bits.ptr, counts.ptr);
       }
       LSAMD_HIP(hipGetLastError());
       return true;
@@ -804,7 +844,8 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
     if (!ok) return false;
     kept = 0;
     if (!exclusive_scan(counts.ptr, q.lanes, offsets.ptr, kept)) return false;
-    if (!dst.allocate(static_cast<size_t>(kept))) return false;
+    if (!(dst_is_scratch ? at_least(dst, static_cast<size_t>(kept)) : dst.allocate(static_cast<size_t>(kept))))
+      return false;
     if (kept > 0) {
       enum_write_kernel<<<nblocks, 256, 0, stream()>>>(q, bits.ptr, offsets.ptr, dst.ptr);
       LSAMD_HIP(hipGetLastError());
@@ -816,11 +857,10 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
   if (use_factored) {
     // Pass 0: translations only, over every candidate.  Pass 1: the
     // factorised test of the survivors (about one candidate in lx ly).
-    DeviceArray<uint64_t> survivors;
     uint64_t nsurv = 0;
-    if (!filter(p, survivors, nsurv, 1)) return false;
+    if (!filter(p, sc.survivors, nsurv, 1, true)) return false;
     EnumParams q = p;
-    q.list = survivors.ptr;
+    q.list = sc.survivors.ptr;
     q.rank0 = 0;
     q.total = static_cast<int64_t>(nsurv);
     q.lanes = lanes_for(q.total, q.per_lane, true);
@@ -838,13 +878,12 @@ bool enumerate_states(EnumerationSpec const &spec, uint64_t lower, uint64_t uppe
     // (nearly all are rejected by the first one or two).  Pass 2: the
     // survivors, now a short ascending list, against the whole group, spread
     // evenly over the lanes.
-    DeviceArray<uint64_t> survivors;
     uint64_t nsurv = 0;
     p.limit = kSurvivorElements;
     p.survive_only = 1;
-    if (!filter(p, survivors, nsurv)) return false;
+    if (!filter(p, sc.survivors, nsurv, 0, true)) return false;
     EnumParams q = p;
-    q.list = survivors.ptr;
+    q.list = sc.survivors.ptr;
     q.rank0 = 0;
     q.total = static_cast<int64_t>(nsurv);
     q.limit = g.size;

@@ -264,6 +264,8 @@ struct EnumerationSpec {
 // Whether enumerate_states may use the factorised filter for groups with a
 // translation factorisation (on by default; the result is the same).
 extern std::atomic<bool> g_factored_enumeration;
+// Frees the enumeration scratch kept between basis builds.
+void release_enumeration_scratch();
 // Writes the ascending representatives into `out`; returns false on error.
 bool enumerate_states(EnumerationSpec const &spec, uint64_t lower,
                       uint64_t upper, DeviceArray<uint64_t> &out,
