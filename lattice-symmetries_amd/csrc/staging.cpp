@@ -9,6 +9,7 @@
 // and the reverse for device-to-host copies.  Small arrays, and a call that
 // finds the buffers in use by another thread, take the plain path.
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -18,9 +19,27 @@
 namespace lsamd {
 namespace {
 
-constexpr size_t kStageBytes = size_t(32) << 20; // per buffer
-constexpr size_t kSmallCopy = size_t(4) << 20;   // below this, one plain copy
+constexpr size_t kDefaultStageMiB = 64; // per buffer: per-chunk synchronisation costs more than the pipeline fill
+constexpr size_t kSmallCopy = size_t(4) << 20; // below this, one plain copy
 constexpr int kMaxCopyThreads = 16;
+
+// LSAMD_STAGING_MIB sets the buffer size (1 to 256 MiB); LSAMD_STAGING=0
+// sends every copy the plain way.
+size_t stage_bytes() {
+  static size_t const b = [] {
+    size_t mib = kDefaultStageMiB;
+    if (char const *e = std::getenv("LSAMD_STAGING_MIB")) mib = std::strtoull(e, nullptr, 10);
+    return std::min<size_t>(256, std::max<size_t>(1, mib)) << 20;
+  }();
+  return b;
+}
+bool staging_enabled() {
+  static bool const on = [] {
+    char const *e = std::getenv("LSAMD_STAGING");
+    return e == nullptr || std::strcmp(e, "0") != 0;
+  }();
+  return on;
+}
 
 // Host threads that split one memcpy between them.  Created once and kept
 // for the life of the process.
@@ -88,7 +107,7 @@ struct Staging {
     if (failed) return false;
     failed = true;
     for (void *&b : buf)
-      if (hipHostMalloc(&b, kStageBytes, hipHostMallocDefault) != hipSuccess) return false;
+      if (hipHostMalloc(&b, stage_bytes(), hipHostMallocDefault) != hipSuccess) return false;
     if (hipStreamCreateWithFlags(&copy, hipStreamNonBlocking) != hipSuccess) return false;
     for (hipEvent_t *e : {&ev[0], &ev[1], &start, &done})
       if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return false;
@@ -119,7 +138,8 @@ bool plain_to_host(void *h, void const *d, size_t bytes) {
 bool copy_to_device(void *d, void const *h, size_t bytes) {
   if (bytes == 0) return true;
   std::unique_lock<std::mutex> lk(g_staging.busy, std::try_to_lock);
-  if (bytes < kSmallCopy || !lk.owns_lock() || !g_staging.init()) return plain_to_device(d, h, bytes);
+  if (bytes < kSmallCopy || !staging_enabled() || !lk.owns_lock() || !g_staging.init())
+    return plain_to_device(d, h, bytes);
   Staging &S = g_staging;
   auto body = [&]() -> bool {
     // earlier work on the engine stream may still read the destination
@@ -127,10 +147,11 @@ bool copy_to_device(void *d, void const *h, size_t bytes) {
     LSAMD_HIP(hipStreamWaitEvent(S.copy, S.start, 0));
     char *const dp = static_cast<char *>(d);
     char const *const hp = static_cast<char const *>(h);
+    size_t const chunk = stage_bytes();
     size_t k = 0;
-    for (size_t off = 0; off < bytes; off += kStageBytes, ++k) {
+    for (size_t off = 0; off < bytes; off += chunk, ++k) {
       int const b = static_cast<int>(k & 1);
-      size_t const len = std::min(kStageBytes, bytes - off);
+      size_t const len = std::min(chunk, bytes - off);
       if (k >= 2) LSAMD_HIP(hipEventSynchronize(S.ev[b])); // its previous chunk has left
       S.pool->copy(S.buf[b], hp + off, len);
       LSAMD_HIP(hipMemcpyAsync(dp + off, S.buf[b], len, hipMemcpyHostToDevice, S.copy));
@@ -149,14 +170,16 @@ bool copy_to_device(void *d, void const *h, size_t bytes) {
 bool copy_to_host(void *h, void const *d, size_t bytes) {
   if (bytes == 0) return true;
   std::unique_lock<std::mutex> lk(g_staging.busy, std::try_to_lock);
-  if (bytes < kSmallCopy || !lk.owns_lock() || !g_staging.init()) return plain_to_host(h, d, bytes);
+  if (bytes < kSmallCopy || !staging_enabled() || !lk.owns_lock() || !g_staging.init())
+    return plain_to_host(h, d, bytes);
   Staging &S = g_staging;
   char const *const dp = static_cast<char const *>(d);
   char *const hp = static_cast<char *>(h);
-  size_t const chunks = (bytes + kStageBytes - 1) / kStageBytes;
+  size_t const chunk = stage_bytes();
+  size_t const chunks = (bytes + chunk - 1) / chunk;
   auto issue = [&](size_t k) -> bool {
     int const b = static_cast<int>(k & 1);
-    size_t const off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    size_t const off = k * chunk, len = std::min(chunk, bytes - off);
     LSAMD_HIP(hipMemcpyAsync(S.buf[b], dp + off, len, hipMemcpyDeviceToHost, S.copy));
     LSAMD_HIP(hipEventRecord(S.ev[b], S.copy));
     return true;
@@ -168,7 +191,7 @@ bool copy_to_host(void *h, void const *d, size_t bytes) {
     if (!issue(0) || (chunks > 1 && !issue(1))) return false;
     for (size_t k = 0; k < chunks; ++k) {
       int const b = static_cast<int>(k & 1);
-      size_t const off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+      size_t const off = k * chunk, len = std::min(chunk, bytes - off);
       LSAMD_HIP(hipEventSynchronize(S.ev[b]));
       S.pool->copy(hp + off, S.buf[b], len);
       if (k + 2 < chunks && !issue(k + 2)) return false;
