@@ -264,11 +264,17 @@ def fp64_product(lib, op, dim, steps, x, y):
     ms = timed(lib, 0, steps, lambda: op.csr_matvec(x, y))
     nnz_off = info["nnz"]
     achieved = (20.0 * nnz_off + 32.0 * dim) / (ms * 1e-3) / 1e9
-    kernel = f"csr_f64_kernel<{sched['lanes_f64']}, {sched_suffix(sched, info['packed'])}>"
+    if info["wide"]:
+        f = sched["flags_f64"]
+        kernel = (f"csr_f64_wide_kernel<{sched['lanes_f64']}, {'true' if f & 1 else 'false'}, "
+                  f"{'true' if f & 2 else 'false'}>")
+    else:
+        kernel = f"csr_f64_kernel<{sched['lanes_f64']}, {sched_suffix(sched, info['packed'])}>"
     if sched["flags_f64"] & 6 == 6:
         kernel += " full grid"
+    table = "packed" if info["packed"] else ("wide" if info["wide"] else None)
     return {"kernel": kernel, "kernel_ms": ms, "off_diagonal_nonzeros": nnz_off, "packed": info["packed"],
-            "value_table": info["table_size"] if info["packed"] else None, "schedule": sched,
+            "table": table, "value_table": info["table_size"] if table else None, "schedule": sched,
             "nonzeros_per_s": (nnz_off + dim) / (ms * 1e-3), "achieved_GBs": achieved,
             "frac": achieved / HBM_PEAK_GBS}
 
@@ -320,9 +326,10 @@ def hdf5_io(basis, x, dim):
 
 
 def general_case(lib, basis, L, dim, steps, rng):
-    """The FP64 product where the value table does not apply: the same 6x6
-    matrix stored unpacked (column and FP64 value per entry), and a model
-    with its own coupling on every bond (too many distinct values to pack)."""
+    """The FP64 product beyond the 256-value table: the same 6x6 matrix
+    stored unpacked (column and FP64 value per entry), and a model with its
+    own coupling on every bond (2836 distinct values), stored with the wide
+    value table (column and 16-bit value index) and unpacked."""
     import lsamd
     from lsamd import models
     x = lsamd.DeviceArray.from_numpy(rng.standard_normal(dim))
@@ -338,15 +345,31 @@ def general_case(lib, basis, L, dim, steps, rng):
     op = lsamd.DeviceOperator(basis, models.random_bond_square(L, seed=7))
     t0 = time.perf_counter()
     op.build_csr(64)
-    out["fp64_random_couplings"] = {
-        "model": f"J1-J2 bonds of the {L}x{L} lattice with seeded couplings J (1 + 0.5 u), u in [-1/2, 1/2) per "
-                 "bond (seed 7), on the same symmetric basis (push-form product of the reference's formula)",
-        "csr_build_seconds": time.perf_counter() - t0, **fp64_product(lib, op, dim, steps, x, y)}
+    model = (f"J1-J2 bonds of the {L}x{L} lattice with seeded couplings J (1 + 0.5 u), u in [-1/2, 1/2) per "
+             "bond (seed 7), on the same symmetric basis (push-form product of the reference's formula)")
+    out["fp64_random_couplings"] = {"model": model, "csr_build_seconds": time.perf_counter() - t0,
+                                    **fp64_product(lib, op, dim, steps, x, y)}
+    want = lsamd.DeviceArray(8 * dim)
+    op.csr_matvec(x, want)
+    sw = op.schedule
     del op
+    op = lsamd.DeviceOperator(basis, models.random_bond_square(L, seed=7))
+    op.set_csr_packing(False)
+    t0 = time.perf_counter()
+    op.build_csr(64)
+    out["fp64_random_couplings_unpacked"] = {"model": model, "csr_build_seconds": time.perf_counter() - t0,
+                                             **fp64_product(lib, op, dim, steps, x, y)}
+    # the two formats store the same values: with the same lanes per row (the
+    # order of each row's sum) the products agree bit for bit
+    op.set_schedule(sw["lanes_f64"], sw["flags_f64"], sw["lanes_f32"], sw["flags_f32"])
+    op.csr_matvec(x, y)
+    out["fp64_random_couplings"]["same_bits_as_unpacked"] = bool(
+        np.array_equal(want.to_numpy(np.float64, (dim,)), y.to_numpy(np.float64, (dim,))))
+    del op, want
     for k, v in out.items():
         traffic, src = profiled_traffic(f"square{L}_general", v["kernel"])
         v.update({"traffic": traffic, "traffic_source": src, **traffic_rate(traffic, v["kernel_ms"])})
-        log(f"{k}: {v['kernel_ms']:.3f} ms, {v['frac']:.3f} of the roofline, packed {v['packed']}")
+        log(f"{k}: {v['kernel_ms']:.3f} ms, {v['frac']:.3f} of the roofline, value table {v['table']}")
     return out
 
 

@@ -320,8 +320,10 @@ int ls_amd_matvec_c128(ls_amd_operator *op, double const *d_x, double *d_y);
 int64_t ls_amd_operator_build_csr(ls_amd_operator *op, int precision);
 /* Value table (default on): a real CSR with at most 256 distinct stored
    values and fewer than 2^24 rows keeps each entry as one 32-bit word,
-   column | value index << 24, and the values in a table (set before
-   ls_amd_operator_build_csr). */
+   column | value index << 24, and the values in a table; one with at most
+   4096 distinct values (any number of rows) keeps the 32-bit column and a
+   16-bit value index (the wide table).  Set before
+   ls_amd_operator_build_csr; 0 keeps (column, value) pairs. */
 int ls_amd_operator_set_csr_packing(ls_amd_operator *op, int enable);
 /* Orbit search of the fused kernels (CSR build and on-the-fly product).
    When the group is the set of translations of an lx x ly torus (site
@@ -333,8 +335,8 @@ int ls_amd_operator_set_csr_packing(ls_amd_operator *op, int enable);
    ls_amd_operator_uses_factored reports which one runs (1 / 0). */
 int ls_amd_operator_set_factored(ls_amd_operator *op, int enable);
 int ls_amd_operator_uses_factored(ls_amd_operator const *op);
-/* out[4]: stored off-diagonal nonzeros, packed (0/1), value-table size,
-   complex (0/1). */
+/* out[4]: stored off-diagonal nonzeros, value table (0 none, 1 packed into
+   the column words, 2 wide), value-table size, complex (0/1). */
 int ls_amd_operator_csr_info(ls_amd_operator const *op, int64_t *out);
 int64_t ls_amd_operator_csr_nnz(ls_amd_operator const *op);
 /* Stored CSR (off-diagonal) rows: row_ptr[n+1] (int64), cols, values

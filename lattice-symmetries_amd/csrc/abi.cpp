@@ -152,7 +152,8 @@ CacheSignature signature_of(ls_hs_operator const *op) {
 
 size_t csr_bytes(Csr const &c) {
   size_t b = static_cast<size_t>(c.rows + 1) * 8 + static_cast<size_t>(c.nnz) * 4;
-  if (!c.packed) b += static_cast<size_t>(c.nnz) * (c.is_complex ? 16 : 8);
+  if (c.wide) b += static_cast<size_t>(c.nnz) * 2;
+  else if (!c.packed) b += static_cast<size_t>(c.nnz) * (c.is_complex ? 16 : 8);
   return b + static_cast<size_t>(c.rows) * 8;
 }
 
@@ -870,6 +871,18 @@ extern "C" int ls_amd_operator_copy_csr(ls_amd_operator const *op, int64_t *row_
   Csr const &c = op->core.csr;
   if (!c.has64) return fail("FP64 CSR has not been built") ? 0 : -1;
   size_t const width = c.is_complex ? 2 : 1;
+  if (c.wide) {
+    std::vector<double> table(static_cast<size_t>(c.table_size));
+    std::vector<uint16_t> idx(static_cast<size_t>(c.nnz));
+    bool const ok = to_host(row_ptr, c.row_ptr.ptr, static_cast<size_t>(c.rows + 1)) &&
+                    to_host(cols, c.cols.ptr, static_cast<size_t>(c.nnz)) &&
+                    to_host(idx.data(), c.vidx.ptr, idx.size()) &&
+                    to_host(table.data(), c.table64.ptr, table.size()) &&
+                    to_host(diag, c.diag64.ptr, static_cast<size_t>(c.rows));
+    if (!ok) return -1;
+    for (int64_t k = 0; k < c.nnz; ++k) values[k] = table[idx[static_cast<size_t>(k)]];
+    return 0;
+  }
   if (c.packed) {
     std::vector<double> table(static_cast<size_t>(c.table_size));
     bool const ok = to_host(row_ptr, c.row_ptr.ptr, static_cast<size_t>(c.rows + 1)) &&
@@ -940,7 +953,7 @@ extern "C" int ls_amd_operator_csr_info(ls_amd_operator const *op, int64_t *out)
   if (op == nullptr || out == nullptr) return fail("null argument") ? 0 : -1;
   Csr const &c = op->core.csr;
   out[0] = c.nnz;
-  out[1] = c.packed ? 1 : 0;
+  out[1] = c.packed ? 1 : (c.wide ? 2 : 0);
   out[2] = c.table_size;
   out[3] = c.is_complex ? 1 : 0;
   return 0;

@@ -673,8 +673,8 @@ __global__ __launch_bounds__(256) void to_f32_kernel(int64_t n, double const *__
 // ---------------------------------------------------------------------------
 
 constexpr uint64_t kEmptySlot = ~uint64_t{0}; // a NaN pattern, never a stored value
-constexpr int kLocalSlots = 512;
-constexpr int kGlobalSlots = 4096;
+constexpr int kLocalSlots = 2 * kWideTableSize; // per-block LDS hash (64 KB)
+constexpr int kGlobalSlots = 4 * kWideTableSize;
 
 __device__ __forceinline__ uint32_t slot_hash(uint64_t k) {
   k ^= k >> 33;
@@ -748,11 +748,32 @@ __global__ __launch_bounds__(256) void pack_columns_kernel(double const *__restr
   cols[k] = (cols[k] & kPackedColumnMask) | (static_cast<uint32_t>(lo) << 24);
 }
 
-// Replaces the value arrays of a real CSR by a value table when it has at
-// most 256 distinct values and fewer than 2^24 rows; vals is the FP64 value
-// array (kept by the caller until this returns).
+// vidx[k] = index of vals[k] in the sorted table (wide value table)
+__global__ __launch_bounds__(256) void pack_wide_kernel(double const *__restrict__ vals, int64_t nnz,
+                                                        double const *__restrict__ table, int size,
+                                                        uint16_t *__restrict__ vidx) {
+  __shared__ double t[kWideTableSize];
+  for (int i = threadIdx.x; i < size; i += blockDim.x) t[i] = table[i];
+  __syncthreads();
+  int64_t const k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  double const v = vals[k];
+  int lo = 0, hi = size - 1;
+  while (lo < hi) {
+    int const mid = (lo + hi) >> 1;
+    if (t[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  vidx[k] = static_cast<uint16_t>(lo);
+}
+
+// Replaces the value arrays of a real CSR by a value table: packed into the
+// column words when it has at most 256 distinct values and fewer than 2^24
+// rows, else a wide table with 16-bit indices when it has at most
+// kWideTableSize.  vals is the FP64 value array (kept by the caller until
+// this returns).
 bool try_pack_values(Csr &c, double const *vals) {
-  if (c.rows >= (int64_t{1} << 24) || c.nnz == 0) return true;
+  if (c.nnz == 0) return true;
   DeviceArray<unsigned long long> table;
   DeviceArray<int> overflow;
   if (!table.allocate(kGlobalSlots) || !overflow.allocate(1)) return false;
@@ -774,18 +795,25 @@ bool try_pack_values(Csr &c, double const *vals) {
       std::memcpy(&v, &k, sizeof(v));
       values.push_back(v);
     }
-  if (values.empty() || values.size() > static_cast<size_t>(kValueTableSize)) return true;
+  bool const fits_word = values.size() <= static_cast<size_t>(kValueTableSize) && c.rows < (int64_t{1} << 24);
+  if (values.empty() || (!fits_word && values.size() > static_cast<size_t>(kWideTableSize))) return true;
   std::sort(values.begin(), values.end());
   std::vector<float> values32(values.size());
   for (size_t i = 0; i < values.size(); ++i) values32[i] = static_cast<float>(values[i]);
   if (!c.table64.upload(values.data(), values.size()) || !c.table32.upload(values32.data(), values32.size()))
     return false;
-  pack_columns_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(vals, c.nnz, c.table64.ptr,
-                                                                  static_cast<int>(values.size()), c.cols.ptr);
+  int const size = static_cast<int>(values.size());
+  if (fits_word) {
+    pack_columns_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(vals, c.nnz, c.table64.ptr, size, c.cols.ptr);
+  } else {
+    if (!c.vidx.allocate(static_cast<size_t>(c.nnz))) return false;
+    pack_wide_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(vals, c.nnz, c.table64.ptr, size, c.vidx.ptr);
+  }
   LSAMD_HIP(hipGetLastError());
   LSAMD_HIP(hipStreamSynchronize(stream()));
-  c.packed = true;
-  c.table_size = static_cast<int>(values.size());
+  c.packed = fits_word;
+  c.wide = !fits_word;
+  c.table_size = size;
   return true;
 }
 
@@ -892,7 +920,7 @@ bool operator_build_csr(OperatorCore &op, int precision) {
   if (op.allow_packing && !try_pack_values(c, vals.ptr)) return false;
   if (precision == 32 || precision == 96) {
     if (!c.diag32.allocate(static_cast<size_t>(n > 0 ? n : 1))) return false;
-    if (!c.packed) {
+    if (!c.has_table()) {
       if (!c.vals32.allocate(static_cast<size_t>(c.nnz > 0 ? c.nnz : 1))) return false;
       if (c.nnz > 0) to_f32_kernel<<<grid_for(c.nnz, 256), 256, 0, stream()>>>(c.nnz, vals.ptr, c.vals32.ptr);
     }
@@ -901,7 +929,7 @@ bool operator_build_csr(OperatorCore &op, int precision) {
     c.has32 = true;
   }
   if (precision == 64 || precision == 96) {
-    if (!c.packed) c.vals64 = std::move(vals);
+    if (!c.has_table()) c.vals64 = std::move(vals);
     c.has64 = true;
   } else {
     c.diag64.reset();
@@ -968,7 +996,11 @@ bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool acc
   int const grid = product_grid(op, n, op.lanes_f64, op.flags_f64);
   bool ok = dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
     return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
-      if (c.packed)
+      if (c.wide)
+        csr_f64_wide_kernel<LPR, NT, DIE><<<grid, 256, sizeof(double) * c.table_size, stream()>>>(
+            n, c.row_ptr.ptr, c.cols.ptr, c.vidx.ptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y,
+            accumulate ? 1 : 0, op.diag.count > 0 ? 1 : 0);
+      else if (c.packed)
         csr_f64_kernel<LPR, NT, DIE, true><<<grid, 256, 0, stream()>>>(
             n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y,
             accumulate ? 1 : 0, op.diag.count > 0 ? 1 : 0);
@@ -995,7 +1027,10 @@ bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_
     return dispatch_lpr(op.lanes_f32, [&]<int LPR>() -> bool {
       int const grid = product_grid(op, n, LPR, op.flags_f32);
       return dispatch_flags(op.flags_f32, [&]<bool NT, bool DIE>() -> bool {
-        if (c.packed)
+        if (c.wide)
+          csr_block_wide_kernel<K, LPR, float, float, NT, DIE><<<grid, 256, sizeof(float) * c.table_size, stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, c.vidx.ptr, c.table32.ptr, c.table_size, c.diag32.ptr, d_x, d_y);
+        else if (c.packed)
           csr_block_kernel<K, LPR, float, float, NT, DIE, true><<<grid, 256, 0, stream()>>>(
               n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table32.ptr, c.table_size, c.diag32.ptr, d_x, d_y);
         else
@@ -1021,7 +1056,11 @@ bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *
     return dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
       int const grid = product_grid(op, n, LPR, op.flags_f64);
       return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
-        if (c.packed)
+        if (c.wide)
+          csr_block_wide_kernel<K, LPR, double, double, NT, DIE><<<grid, 256, sizeof(double) * c.table_size,
+                                                                   stream()>>>(
+              n, c.row_ptr.ptr, c.cols.ptr, c.vidx.ptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y);
+        else if (c.packed)
           csr_block_kernel<K, LPR, double, double, NT, DIE, true><<<grid, 256, 0, stream()>>>(
               n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y);
         else
@@ -1049,8 +1088,15 @@ int operator_autotune(OperatorCore &op) {
   if (n == 0 || c.is_complex) return op.lanes_f64;
   hipDeviceProp_t prop;
   int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-    op.persistent_blocks = std::max(8, (prop.multiProcessorCount * 4 / 8) * 8);
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    // four workgroups per CU; with the wide table as many as fit beside their
+    // LDS copies of it (up to eight), so each CU stages the table a few times
+    int per_cu = 4;
+    if (c.wide)
+      per_cu = static_cast<int>(std::min<size_t>(
+          8, std::max<size_t>(1, (160u << 10) / (sizeof(double) * static_cast<size_t>(c.table_size)))));
+    op.persistent_blocks = std::max(8, (prop.multiProcessorCount * per_cu / 8) * 8);
+  }
   int const cand[] = {2, 4, 8, 16, 32, 64};
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);

@@ -41,11 +41,16 @@ struct Csr {
   bool is_complex = false; // values are (re, im) pairs of M[i, j]
   // Value table: when the matrix has at most 256 distinct off-diagonal
   // values and fewer than 2^24 rows, cols[k] holds column | (value index
-  // << 24) and vals64 / vals32 are replaced by table64 / table32.
+  // << 24) and vals64 / vals32 are replaced by table64 / table32.  Wide
+  // table (at most kWideTableSize values, any number of rows): cols[k] is the
+  // column, vidx[k] the value's index in table64 / table32.
   bool packed = false;
+  bool wide = false;
   int table_size = 0;
   DeviceArray<double> table64;
   DeviceArray<float> table32;
+  DeviceArray<uint16_t> vidx;
+  bool has_table() const { return packed || wide; }
 };
 
 struct OperatorCore {

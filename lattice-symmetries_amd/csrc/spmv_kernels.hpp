@@ -13,6 +13,11 @@ namespace lsamd {
 // values and fewer than 2^24 rows stores column | value index << 24 per entry.
 constexpr uint32_t kPackedColumnMask = (1u << 24) - 1;
 constexpr int kValueTableSize = 256;
+// Wide value table: at most kWideTableSize distinct values (any number of
+// rows) store the plain column and a 16-bit value index per entry; the table
+// is staged in LDS (dynamic shared memory of its own size, at most 32 KB in
+// FP64).
+constexpr int kWideTableSize = 4096;
 
 // Stream loads of the CSR arrays, optionally non-temporal so that the
 // stream does not displace the gathered vector from L2.
@@ -43,44 +48,49 @@ template <bool DIE> __device__ __forceinline__ void tile_range(int64_t tiles, in
   }
 }
 
-// One stored entry: (column, value), from separate arrays or from a packed
-// column word and the value table in LDS.
-template <bool PACKED, bool NT, typename T>
-__device__ __forceinline__ void csr_entry(uint32_t const *__restrict__ cols, T const *__restrict__ vals,
-                                          T const *__restrict__ table, int64_t k, uint32_t &col, T &v) {
-  if constexpr (PACKED) {
-    uint32_t const p = ld_stream<NT>(cols + k);
-    col = p & kPackedColumnMask;
-    v = table[p >> 24];
-  } else {
+// Readers of stored entry k: (column, value) from separate arrays, from a
+// packed column word and the value table, or from the column, a 16-bit
+// index and the wide value table.  The tables are in LDS.
+template <bool NT, typename T> struct PlainEntries {
+  uint32_t const *__restrict__ cols;
+  T const *__restrict__ vals;
+  __device__ __forceinline__ void operator()(int64_t k, uint32_t &col, T &v) const {
     col = ld_stream<NT>(cols + k);
     v = ld_stream<NT>(vals + k);
   }
-}
-
-// Stages the value table in LDS (a no-op for unpacked matrices).
-template <bool PACKED, typename T>
-__device__ __forceinline__ T const *stage_table(T *lds, T const *__restrict__ table, int size) {
-  if constexpr (PACKED) {
-    for (int i = threadIdx.x; i < size; i += blockDim.x) lds[i] = table[i];
-    __syncthreads();
-    return lds;
-  } else {
-    return nullptr;
+};
+template <bool NT, typename T> struct PackedEntries {
+  uint32_t const *__restrict__ cols;
+  T const *__restrict__ table;
+  __device__ __forceinline__ void operator()(int64_t k, uint32_t &col, T &v) const {
+    uint32_t const p = ld_stream<NT>(cols + k);
+    col = p & kPackedColumnMask;
+    v = table[p >> 24];
   }
+};
+template <bool NT, typename T> struct WideEntries {
+  uint32_t const *__restrict__ cols;
+  uint16_t const *__restrict__ vidx;
+  T const *__restrict__ table;
+  __device__ __forceinline__ void operator()(int64_t k, uint32_t &col, T &v) const {
+    col = ld_stream<NT>(cols + k);
+    v = table[ld_stream<NT>(vidx + k)];
+  }
+};
+
+// Stages `size` table values in LDS for the workgroup.
+template <typename T> __device__ __forceinline__ T const *stage_table(T *lds, T const *__restrict__ table, int size) {
+  for (int i = threadIdx.x; i < size; i += blockDim.x) lds[i] = table[i];
+  __syncthreads();
+  return lds;
 }
 
-template <int LPR, bool NT, bool DIE, bool PACKED>
-__global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *__restrict__ rp,
-                                                      uint32_t const *__restrict__ cols,
-                                                      double const *__restrict__ vals,
-                                                      double const *__restrict__ table, int table_size,
-                                                      double const *__restrict__ diag,
-                                                      double const *__restrict__ x,
-                                                      double *__restrict__ y, int accumulate,
-                                                      int has_diag) {
-  __shared__ double lds_table[PACKED ? kValueTableSize : 1];
-  double const *tab = stage_table<PACKED>(lds_table, table, table_size);
+// y[row] = diag[row] x[row] (or y[row], or 0) + sum of the row's entries
+// v x[col], in entry order, LPR lanes to a row.
+template <int LPR, bool DIE, typename Entries>
+__device__ __forceinline__ void csr_f64_rows(int64_t n, int64_t const *__restrict__ rp, Entries const &ent,
+                                             double const *__restrict__ diag, double const *__restrict__ x,
+                                             double *__restrict__ y, int accumulate, int has_diag) {
   constexpr int kRows = 256 / LPR;
   int64_t const tiles = (n + kRows - 1) / kRows;
   int64_t t, t_end, t_step;
@@ -97,10 +107,10 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
       for (; k + 3 * LPR < e; k += 4 * LPR) {
         uint32_t c0, c1, c2, c3;
         double v0, v1, v2, v3;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, c0, v0);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + LPR, c1, v1);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + 2 * LPR, c2, v2);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + 3 * LPR, c3, v3);
+        ent(k, c0, v0);
+        ent(k + LPR, c1, v1);
+        ent(k + 2 * LPR, c2, v2);
+        ent(k + 3 * LPR, c3, v3);
         double const x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
         acc = fma(v0, x0, acc);
         acc = fma(v1, x1, acc);
@@ -110,7 +120,7 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
       for (; k < e; k += LPR) {
         uint32_t col;
         double v;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
+        ent(k, col, v);
         acc = fma(v, x[col], acc);
       }
     }
@@ -123,16 +133,46 @@ __global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *
   }
 }
 
-template <int K, int LPR, typename T, typename TO, bool NT, bool DIE, bool PACKED>
-__global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const *__restrict__ rp,
-                                                        uint32_t const *__restrict__ cols,
-                                                        T const *__restrict__ vals,
-                                                        T const *__restrict__ table, int table_size,
-                                                        T const *__restrict__ diag,
-                                                        T const *__restrict__ x,
-                                                        TO *__restrict__ y) {
-  __shared__ T lds_table[PACKED ? kValueTableSize : 1];
-  T const *tab = stage_table<PACKED>(lds_table, table, table_size);
+template <int LPR, bool NT, bool DIE, bool PACKED>
+__global__ __launch_bounds__(256) void csr_f64_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                      uint32_t const *__restrict__ cols,
+                                                      double const *__restrict__ vals,
+                                                      double const *__restrict__ table, int table_size,
+                                                      double const *__restrict__ diag,
+                                                      double const *__restrict__ x,
+                                                      double *__restrict__ y, int accumulate,
+                                                      int has_diag) {
+  if constexpr (PACKED) {
+    __shared__ double lds_table[kValueTableSize];
+    PackedEntries<NT, double> const ent{cols, stage_table(lds_table, table, table_size)};
+    csr_f64_rows<LPR, DIE>(n, rp, ent, diag, x, y, accumulate, has_diag);
+  } else {
+    PlainEntries<NT, double> const ent{cols, vals};
+    csr_f64_rows<LPR, DIE>(n, rp, ent, diag, x, y, accumulate, has_diag);
+  }
+}
+
+template <int LPR, bool NT, bool DIE>
+__global__ __launch_bounds__(256) void csr_f64_wide_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                           uint32_t const *__restrict__ cols,
+                                                           uint16_t const *__restrict__ vidx,
+                                                           double const *__restrict__ table, int table_size,
+                                                           double const *__restrict__ diag,
+                                                           double const *__restrict__ x,
+                                                           double *__restrict__ y, int accumulate,
+                                                           int has_diag) {
+  extern __shared__ unsigned char wide_lds[]; // table_size doubles (dynamic)
+  WideEntries<NT, double> const ent{cols, vidx,
+                                    stage_table(reinterpret_cast<double *>(wide_lds), table, table_size)};
+  csr_f64_rows<LPR, DIE>(n, rp, ent, diag, x, y, accumulate, has_diag);
+}
+
+// y[row][q] = diag[row] x[row][q] + sum of the row's entries v x[col][q] for
+// K right-hand sides stored [rows][K], summed in T.
+template <int K, int LPR, typename T, typename TO, bool DIE, typename Entries>
+__device__ __forceinline__ void csr_block_rows(int64_t n, int64_t const *__restrict__ rp, Entries const &ent,
+                                               T const *__restrict__ diag, T const *__restrict__ x,
+                                               TO *__restrict__ y) {
   constexpr int kRows = 256 / LPR;
   int64_t const tiles = (n + kRows - 1) / kRows;
   int64_t t, t_end, t_step;
@@ -150,8 +190,8 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const
       for (; k + LPR < e; k += 2 * LPR) {
         uint32_t c0, c1;
         T v0, v1;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, c0, v0);
-        csr_entry<PACKED, NT>(cols, vals, tab, k + LPR, c1, v1);
+        ent(k, c0, v0);
+        ent(k + LPR, c1, v1);
         T const *x0 = x + static_cast<size_t>(c0) * K;
         T const *x1 = x + static_cast<size_t>(c1) * K;
         T a0[K], a1[K];
@@ -168,7 +208,7 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const
       for (; k < e; k += LPR) {
         uint32_t col;
         T v;
-        csr_entry<PACKED, NT>(cols, vals, tab, k, col, v);
+        ent(k, col, v);
         T const *xr = x + static_cast<size_t>(col) * K;
 #pragma unroll
         for (int q = 0; q < K; ++q) acc[q] = fma(v, xr[q], acc[q]);
@@ -185,6 +225,37 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const
         y[static_cast<size_t>(row) * K + q] = static_cast<TO>(d * x[static_cast<size_t>(row) * K + q] + acc[q]);
     }
   }
+}
+
+template <int K, int LPR, typename T, typename TO, bool NT, bool DIE, bool PACKED>
+__global__ __launch_bounds__(256) void csr_block_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                        uint32_t const *__restrict__ cols,
+                                                        T const *__restrict__ vals,
+                                                        T const *__restrict__ table, int table_size,
+                                                        T const *__restrict__ diag,
+                                                        T const *__restrict__ x,
+                                                        TO *__restrict__ y) {
+  if constexpr (PACKED) {
+    __shared__ T lds_table[kValueTableSize];
+    PackedEntries<NT, T> const ent{cols, stage_table(lds_table, table, table_size)};
+    csr_block_rows<K, LPR, T, TO, DIE>(n, rp, ent, diag, x, y);
+  } else {
+    PlainEntries<NT, T> const ent{cols, vals};
+    csr_block_rows<K, LPR, T, TO, DIE>(n, rp, ent, diag, x, y);
+  }
+}
+
+template <int K, int LPR, typename T, typename TO, bool NT, bool DIE>
+__global__ __launch_bounds__(256) void csr_block_wide_kernel(int64_t n, int64_t const *__restrict__ rp,
+                                                             uint32_t const *__restrict__ cols,
+                                                             uint16_t const *__restrict__ vidx,
+                                                             T const *__restrict__ table, int table_size,
+                                                             T const *__restrict__ diag,
+                                                             T const *__restrict__ x,
+                                                             TO *__restrict__ y) {
+  extern __shared__ unsigned char wide_lds[]; // table_size values of T (dynamic)
+  WideEntries<NT, T> const ent{cols, vidx, stage_table(reinterpret_cast<T *>(wide_lds), table, table_size)};
+  csr_block_rows<K, LPR, T, TO, DIE>(n, rp, ent, diag, x, y);
 }
 
 } // namespace lsamd
