@@ -364,7 +364,9 @@ int ls_amd_operator_is_hermitian(ls_amd_operator *op, int mode);
    flags: bit 0 non-temporal loads of the CSR stream, bit 1 die-aware
    mapping (each accelerator die's workgroups walk one contiguous range of
    rows) on the persistent grid, bit 2 (with bit 1) the die-aware mapping
-   over one workgroup per tile instead. */
+   over one workgroup per tile instead.  Unless set (ls_amd_operator_set_schedule)
+   or autotuned, the CSR build picks lanes so that each lane sums about four
+   entries of a mean row, and flags 6 from a million rows on (else 0). */
 int ls_amd_operator_schedule(ls_amd_operator const *op, int *out);
 /* Process-wide: whether basis builds may use the translation-factorised
    filter for groups of torus translations times point elements with

@@ -623,13 +623,18 @@ extern "C" void ls_chpl_matrix_vector_product(ls_hs_operator *matrixPtr, int num
   bool ok = (x.count == static_cast<size_t>(n) || x.allocate(n)) &&
             (y.count == static_cast<size_t>(n) || y.allocate(n)) && copy_to_device(x.ptr, xPtr, bytes) &&
             (has_diag || copy_to_device(y.ptr, yPtr, bytes));
-  if (ok) {
-    if (c->use_csr)
-      ok = operator_csr_f64(core, x.ptr, y.ptr, !has_diag);
-    else
-      ok = operator_matvec_otf(core, x.ptr, y.ptr, !has_diag, false);
+  if (ok && c->use_csr) {
+    // y = diag x (or the caller's y), then the rows in chunks, each chunk of
+    // y copied to the host while the next one is computed
+    auto rows = [&](size_t first, size_t count) -> bool {
+      return operator_csr_f64_rows(core, x.ptr, y.ptr, static_cast<int64_t>(first / sizeof(double)),
+                                   static_cast<int64_t>((first + count) / sizeof(double)));
+    };
+    ok = (!has_diag || operator_csr_f64_diag(core, x.ptr, y.ptr)) &&
+         produce_to_host(yPtr, y.ptr, bytes, sizeof(double), rows);
+  } else if (ok) {
+    ok = operator_matvec_otf(core, x.ptr, y.ptr, !has_diag, false) && copy_to_host(yPtr, y.ptr, bytes);
   }
-  ok = ok && copy_to_host(yPtr, y.ptr, bytes);
   if (!ok) raise_last();
 }
 
@@ -985,5 +990,6 @@ extern "C" int ls_amd_operator_set_schedule(ls_amd_operator *op, int lanes_f64, 
   op->core.flags_f64 = flags_f64;
   op->core.lanes_f32 = lanes_f32;
   op->core.flags_f32 = flags_f32;
+  op->core.schedule_set = true;
   return 0;
 }

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <functional>
 #include <cstdint>
 #include <cstdio>
 #include <memory>
@@ -47,6 +48,13 @@ hipStream_t stream();
 // for the work queued on stream(), returns when h holds the data.
 bool copy_to_device(void *d, void const *h, size_t bytes);
 bool copy_to_host(void *h, void const *d, size_t bytes);
+// Copies `bytes` of d to h while they are produced: produce(first, count)
+// enqueues on stream() the work that writes bytes [first, first + count) of
+// d, for consecutive chunks (multiples of `unit` bytes), and each chunk goes
+// to the host as soon as its work has run.  Without the staging buffers all
+// chunks are produced first and copied in one piece.
+bool produce_to_host(void *h, void const *d, size_t bytes, size_t unit,
+                     std::function<bool(size_t, size_t)> const &produce);
 
 // Kernel timing by family (see ls_amd_last_kernel_ms).
 enum TimerFamily {

@@ -817,6 +817,22 @@ bool try_pack_values(Csr &c, double const *vals) {
   return true;
 }
 
+// The product schedule when neither the caller nor the autotuner chose one:
+// lanes per row so that each lane sums about four entries of a mean row, and
+// for a million rows or more the die-aware mapping with one workgroup per
+// tile, the autotuner's choice for such matrices (16 lanes and that mapping
+// on 6x6, against 8 lanes on the plain grid otherwise).
+void default_schedule(OperatorCore &op) {
+  Csr const &c = op.csr;
+  double const mean = c.rows > 0 ? static_cast<double>(c.nnz) / static_cast<double>(c.rows) : 0.0;
+  int lanes = 2;
+  while (lanes < 64 && 8.0 * lanes <= mean) lanes *= 2;
+  op.lanes_f64 = lanes;
+  op.flags_f64 = c.rows >= (int64_t{1} << 20) ? 6 : 0;
+  op.lanes_f32 = lanes;
+  op.flags_f32 = 0;
+}
+
 bool operator_build_csr(OperatorCore &op, int precision) {
   if (precision != 32 && precision != 64 && precision != 96 && precision != 128)
     return fail("precision must be 32, 64, 96 (real) or 128 (complex)");
@@ -934,6 +950,7 @@ bool operator_build_csr(OperatorCore &op, int precision) {
   } else {
     c.diag64.reset();
   }
+  if (!op.schedule_set) default_schedule(op);
   LSAMD_HIP(hipStreamSynchronize(stream()));
   return true;
 }
@@ -986,6 +1003,29 @@ template <typename F> bool dispatch_k(int k, F &&f) {
   }
 }
 
+// The FP64 CSR product over `n` rows starting at row_ptr `rp` and output
+// `y` (offsets of the whole arrays; x and the columns stay global).
+bool launch_csr_f64(OperatorCore &op, int64_t const *rp, int64_t n, double const *d_x, double *d_y, int accumulate,
+                    int has_diag) {
+  Csr const &c = op.csr;
+  int const grid = product_grid(op, n, op.lanes_f64, op.flags_f64);
+  return dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
+    return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
+      if (c.wide)
+        csr_f64_wide_kernel<LPR, NT, DIE><<<grid, 256, sizeof(double) * c.table_size, stream()>>>(
+            n, rp, c.cols.ptr, c.vidx.ptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y, accumulate, has_diag);
+      else if (c.packed)
+        csr_f64_kernel<LPR, NT, DIE, true><<<grid, 256, 0, stream()>>>(
+            n, rp, c.cols.ptr, nullptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y, accumulate, has_diag);
+      else
+        csr_f64_kernel<LPR, NT, DIE, false><<<grid, 256, 0, stream()>>>(
+            n, rp, c.cols.ptr, c.vals64.ptr, nullptr, 0, c.diag64.ptr, d_x, d_y, accumulate, has_diag);
+      LSAMD_HIP(hipGetLastError());
+      return true;
+    });
+  });
+}
+
 bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool accumulate) {
   Csr const &c = op.csr;
   if (c.is_complex) return fail("the CSR holds complex values: use a sector batch for its products");
@@ -993,27 +1033,32 @@ bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool acc
   int64_t const n = c.rows;
   if (n == 0) return true;
   timer_begin(kTimerCsrF64);
-  int const grid = product_grid(op, n, op.lanes_f64, op.flags_f64);
-  bool ok = dispatch_lpr(op.lanes_f64, [&]<int LPR>() -> bool {
-    return dispatch_flags(op.flags_f64, [&]<bool NT, bool DIE>() -> bool {
-      if (c.wide)
-        csr_f64_wide_kernel<LPR, NT, DIE><<<grid, 256, sizeof(double) * c.table_size, stream()>>>(
-            n, c.row_ptr.ptr, c.cols.ptr, c.vidx.ptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y,
-            accumulate ? 1 : 0, op.diag.count > 0 ? 1 : 0);
-      else if (c.packed)
-        csr_f64_kernel<LPR, NT, DIE, true><<<grid, 256, 0, stream()>>>(
-            n, c.row_ptr.ptr, c.cols.ptr, nullptr, c.table64.ptr, c.table_size, c.diag64.ptr, d_x, d_y,
-            accumulate ? 1 : 0, op.diag.count > 0 ? 1 : 0);
-      else
-        csr_f64_kernel<LPR, NT, DIE, false><<<grid, 256, 0, stream()>>>(
-            n, c.row_ptr.ptr, c.cols.ptr, c.vals64.ptr, nullptr, 0, c.diag64.ptr, d_x, d_y, accumulate ? 1 : 0,
-            op.diag.count > 0 ? 1 : 0);
-      LSAMD_HIP(hipGetLastError());
-      return true;
-    });
-  });
+  bool const ok = launch_csr_f64(op, c.row_ptr.ptr, n, d_x, d_y, accumulate ? 1 : 0, op.diag.count > 0 ? 1 : 0);
   timer_end(kTimerCsrF64);
   return ok;
+}
+
+__global__ __launch_bounds__(256) void diag_times_kernel(int64_t n, double const *__restrict__ diag,
+                                                         double const *__restrict__ x, double *__restrict__ y) {
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) y[i] = diag[i] * x[i];
+}
+
+bool operator_csr_f64_diag(OperatorCore &op, double const *d_x, double *d_y) {
+  Csr const &c = op.csr;
+  if (c.is_complex || !c.has64) return fail("FP64 CSR has not been built");
+  if (c.rows == 0) return true;
+  diag_times_kernel<<<grid_for(c.rows, 256), 256, 0, stream()>>>(c.rows, c.diag64.ptr, d_x, d_y);
+  LSAMD_HIP(hipGetLastError());
+  return true;
+}
+
+bool operator_csr_f64_rows(OperatorCore &op, double const *d_x, double *d_y, int64_t r0, int64_t r1) {
+  Csr const &c = op.csr;
+  if (c.is_complex || !c.has64) return fail("FP64 CSR has not been built");
+  if (r0 < 0 || r1 > c.rows || r0 > r1) return fail("row range outside the CSR");
+  if (r1 == r0) return true;
+  return launch_csr_f64(op, c.row_ptr.ptr + r0, r1 - r0, d_x, d_y + r0, 1, 0);
 }
 
 bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_y) {
@@ -1086,6 +1131,7 @@ int operator_autotune(OperatorCore &op) {
   Csr const &c = op.csr;
   int64_t const n = c.rows;
   if (n == 0 || c.is_complex) return op.lanes_f64;
+  op.schedule_set = true;
   hipDeviceProp_t prop;
   int dev = 0;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {

@@ -73,6 +73,7 @@ struct OperatorCore {
   int flags_f64 = 0;  // bit 0 non-temporal stream loads, bit 1 die-aware persistent grid
   int flags_f32 = 0;
   int persistent_blocks = 1024; // grid of the die-aware schedule (a multiple of 8)
+  bool schedule_set = false;    // set by the caller or the autotuner: the CSR build keeps it
   bool allow_packing = true;    // value-table CSR when possible
   bool allow_factored = true;   // factorised orbit search when the group has one
   DeviceArray<int> d_err; // error flag + offending state
@@ -86,6 +87,12 @@ bool operator_matvec_otf(OperatorCore &op, double const *d_x, double *d_y, bool 
                          bool complex_values);
 bool operator_build_csr(OperatorCore &op, int precision);
 bool operator_csr_f64(OperatorCore &op, double const *d_x, double *d_y, bool accumulate);
+// The same product in pieces, for overlapping it with copies of y: first
+// y = diag x over all rows (operator_csr_f64_diag, when the operator has
+// diagonal terms), then y[row] += the row's sum for rows [r0, r1)
+// (operator_csr_f64_rows).  Each row gets the bits operator_csr_f64 gives it.
+bool operator_csr_f64_diag(OperatorCore &op, double const *d_x, double *d_y);
+bool operator_csr_f64_rows(OperatorCore &op, double const *d_x, double *d_y, int64_t r0, int64_t r1);
 bool operator_csr_f32_block(OperatorCore &op, int k, float const *d_x, float *d_y);
 bool operator_csr_f64_block(OperatorCore &op, int k, double const *d_x, double *d_y);
 int operator_autotune(OperatorCore &op);

@@ -99,7 +99,18 @@ struct Staging {
   hipStream_t copy = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipEvent_t start = nullptr, done = nullptr;
+  std::vector<hipEvent_t> produced; // one per chunk of produce_to_host (grow-only)
   CopyPool *pool = nullptr;
+
+  // Called with `busy` held.
+  bool chunk_events(size_t n) {
+    while (produced.size() < n) {
+      hipEvent_t e;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+      produced.push_back(e);
+    }
+    return true;
+  }
 
   // Called with `busy` held.
   bool init() {
@@ -200,6 +211,52 @@ bool copy_to_host(void *h, void const *d, size_t bytes) {
   };
   bool const ok = body();
   bool const drained = hipStreamSynchronize(S.copy) == hipSuccess;
+  return ok && (drained || fail("staging copy stream failed"));
+}
+
+bool produce_to_host(void *h, void const *d, size_t bytes, size_t unit,
+                     std::function<bool(size_t, size_t)> const &produce) {
+  if (bytes == 0) return true;
+  unit = std::max<size_t>(1, unit);
+  std::unique_lock<std::mutex> lk(g_staging.busy, std::try_to_lock);
+  size_t const chunk = stage_bytes() / unit * unit;
+  size_t const chunks = chunk == 0 ? 0 : (bytes + chunk - 1) / chunk;
+  bool const staged = bytes >= kSmallCopy && staging_enabled() && lk.owns_lock() && chunk > 0 &&
+                      g_staging.init() && g_staging.chunk_events(chunks);
+  if (!staged) {
+    if (!produce(0, bytes)) return false;
+    return plain_to_host(h, d, bytes);
+  }
+  Staging &S = g_staging;
+  char const *const dp = static_cast<char const *>(d);
+  char *const hp = static_cast<char *>(h);
+  auto issue = [&](size_t k) -> bool {
+    int const b = static_cast<int>(k & 1);
+    size_t const off = k * chunk, len = std::min(chunk, bytes - off);
+    LSAMD_HIP(hipStreamWaitEvent(S.copy, S.produced[k], 0));
+    LSAMD_HIP(hipMemcpyAsync(S.buf[b], dp + off, len, hipMemcpyDeviceToHost, S.copy));
+    LSAMD_HIP(hipEventRecord(S.ev[b], S.copy));
+    return true;
+  };
+  auto body = [&]() -> bool {
+    // all the producing work is queued first, each chunk's end marked
+    for (size_t k = 0; k < chunks; ++k) {
+      size_t const off = k * chunk;
+      if (!produce(off, std::min(chunk, bytes - off))) return false;
+      LSAMD_HIP(hipEventRecord(S.produced[k], stream()));
+    }
+    if (!issue(0) || (chunks > 1 && !issue(1))) return false;
+    for (size_t k = 0; k < chunks; ++k) {
+      int const b = static_cast<int>(k & 1);
+      size_t const off = k * chunk, len = std::min(chunk, bytes - off);
+      LSAMD_HIP(hipEventSynchronize(S.ev[b]));
+      S.pool->copy(hp + off, S.buf[b], len);
+      if (k + 2 < chunks && !issue(k + 2)) return false;
+    }
+    return true;
+  };
+  bool const ok = body();
+  bool const drained = hipStreamSynchronize(S.copy) == hipSuccess && hipStreamSynchronize(stream()) == hipSuccess;
   return ok && (drained || fail("staging copy stream failed"));
 }
 

@@ -90,3 +90,16 @@ def test_two_threads_at_once(setup):
     for k in range(2):
         want = reference_product(dop, xs[k], np.zeros(n))
         assert np.abs(ys[k] - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_chunked_product_has_the_bits_of_one_launch(setup):
+    """The reference entry point runs its CSR product in row chunks, each
+    chunk of y copied to the host while the next is computed; every row gets
+    the bits of the single-launch product under the same schedule."""
+    hop, dop, n = setup
+    csr = lsamd.DeviceOperator(dop.basis, xy_chain(N))
+    csr.build_csr(64)  # the same default schedule as the cached operator
+    x = np.random.default_rng(13).standard_normal(n)
+    y = np.zeros(n)
+    hop.matvec(x, out=y)
+    assert np.array_equal(y, csr.matvec(x, "csr"))
