@@ -461,6 +461,10 @@ def configs_bench(args):
                                   "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                                   **traffic_rate(traffic, k_ms), "kernel": kernel,
                                   "kernel_ms": k_ms, "algorithmic_bytes_per_launch": bytes_per_launch}})
+        if bytes_per_launch < 64e6:
+            # a product this small finishes within a few microseconds of its launch
+            line["roofline"]["note"] = (f"{bytes_per_launch / 1e3:.0f} KB per launch: the product is bound by launch "
+                                        "latency, not memory; the fraction does not rate the kernel")
         log(f"{name}: {dim} states in {t_build:.3f} s, {nnz_off} off-diagonal nonzeros, product {k_ms:.3f} ms "
             f"({achieved / HBM_PEAK_GBS:.2f} of the roofline), on the fly {otf_ms:.3f} ms")
         print(json.dumps(line), flush=True)
