@@ -70,3 +70,22 @@ def test_wide_table_is_chosen_and_stores_the_same_matrix(setup):
             assert np.abs(y - want).max() <= RTOL64 * np.abs(want).max()
     wide.autotune()
     assert np.abs(wide.matvec(x, "csr") - want).max() <= RTOL64 * np.abs(want).max()
+
+
+def test_wide_table_serves_more_than_2_to_24_rows(engine):
+    """A matrix with few distinct values but 2^24 rows or more cannot pack
+    its index into the column word: the unprojected 28-site chain (40 116 600
+    rows, a handful of values) takes the wide table, and its CSR product
+    matches the on-the-fly product (an independent path, no CSR)."""
+    n, h = 28, 14
+    db = lsamd.DeviceBasis(n, h)
+    assert db.build() == 40116600
+    op = lsamd.DeviceOperator(db, models.heisenberg_chain(n, 1.0, 0.5))
+    op.build_csr(64)
+    info = op.csr_info
+    say("28-site chain csr info", info)
+    assert info["wide"] and info["table_size"] <= 256
+    x = np.random.default_rng(9).standard_normal(40116600)
+    y = op.matvec(x, "csr")
+    want = op.matvec(x, "otf")
+    assert np.abs(y - want).max() <= RTOL64 * np.abs(want).max()
