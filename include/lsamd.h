@@ -361,6 +361,11 @@ int ls_amd_operator_autotune(ls_amd_operator *op);
 int ls_amd_operator_is_hermitian(ls_amd_operator *op, int mode);
 /* Current CSR product schedule: out[5] = lanes per row (FP64), flags (FP64),
    lanes per row (FP32 block), flags (FP32 block), persistent grid size.
+   The FP64 pair serves every product that gathers at most 16 bytes of x per
+   stored entry (the FP64 vector, FP64 blocks of 1-2, FP32 blocks of 1-4),
+   the FP32 pair the wider blocks (FP32 of 8-16, FP64 of 4-16); the
+   autotuner times the first on the FP64 vector, the second on the FP32
+   block of 8.
    flags: bit 0 non-temporal loads of the CSR stream, bit 1 die-aware
    mapping (each accelerator die's workgroups walk one contiguous range of
    rows) on the persistent grid, bit 2 (with bit 1) the die-aware mapping

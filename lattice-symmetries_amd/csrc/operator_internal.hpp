@@ -68,8 +68,11 @@ struct OperatorCore {
   Terms diag, off;
   // CSR cache
   Csr csr;
+  // The FP64 schedule serves every product that gathers at most 16 B of x
+  // per entry (the FP64 vector, FP64 blocks of 2, FP32 blocks up to 4), the
+  // FP32 one the wider blocks (FP32 from 8 columns, FP64 from 4).
   int lanes_f64 = 8;  // lanes per row of the FP64 SpMV
-  int lanes_f32 = 8;  // lanes per row of the FP32 block SpMV
+  int lanes_f32 = 8;  // lanes per row of the FP32 block-of-8 SpMV
   int flags_f64 = 0;  // bit 0 non-temporal stream loads, bit 1 die-aware persistent grid
   int flags_f32 = 0;
   int persistent_blocks = 1024; // grid of the die-aware schedule (a multiple of 8)

@@ -266,6 +266,40 @@ def test_every_csr_schedule_gives_the_same_product(engine):
         dop.set_schedule(3, 0, 8, 0)
 
 
+def test_block_products_take_the_schedule_of_their_gather_width(engine):
+    """Block products that gather at most 16 B of x per stored entry (FP32
+    blocks of 1, 2, 4; FP64 blocks of 1, 2) run on the FP64 schedule, wider
+    ones on the FP32 schedule.  The lanes per row fix the order of a row's
+    sum, so with the two schedules at 1 and 64 lanes each block size must
+    give the bits of the schedule it belongs to, and match the oracle."""
+    n, h = 20, 10
+    gens = [models.chain_translation(n), models.chain_parity(n)]
+    S = symmetries(gens, [0, 1])
+    expr = models.j1j2_chain(n, 1.0, 0.5)
+    og = O.Group(n, S.elements(), S.characters.real, S.characters.imag, -1)
+    db = lsamd.DeviceBasis(n, h, -1, S)
+    db.build()
+    reps = db.states
+    diag, off = expr.split()
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal((reps.size, 16))
+    ref = np.stack([O.matvec_push(og, n, -1, reps, O.Terms(diag), O.Terms(off), X[:, q]) for q in range(16)], 1)
+    dop = lsamd.DeviceOperator(db, expr)
+    dop.build_csr(96)
+    for prec, narrow_ks, wide_ks, tol in ((32, (1, 2, 4), (8, 16), 2e-5), (64, (1, 2), (4, 8, 16), RTOL64)):
+        for k in narrow_ks + wide_ks:
+            Xk = X[:, :k]
+            dop.set_schedule(1, 7, 64, 0)
+            mixed = dop.matvec_block(Xk, prec)
+            assert np.abs(mixed - ref[:, :k]).max() <= tol * np.abs(ref[:, :k]).max(), (prec, k)
+            lanes, flags = (1, 7) if k in narrow_ks else (64, 0)
+            dop.set_schedule(lanes, flags, lanes, flags)
+            same = dop.matvec_block(Xk, prec)
+            say("block schedule", prec, k, lanes, flags)
+            assert np.array_equal(mixed.view(np.uint32 if prec == 32 else np.uint64),
+                                  same.view(np.uint32 if prec == 32 else np.uint64)), (prec, k)
+
+
 def test_matvec_rejects_operator_breaking_symmetry(engine):
     say("error path")
     n, h = 12, 6
