@@ -49,8 +49,10 @@ double seconds_since(Clock::time_point t) { return std::chrono::duration<double>
 constexpr int kChunkRows = 2048;
 
 // A block step re-orthogonalises once, and a second time only when the first
-// pass removed all but kReorthDrop of some column's squared norm: then one
-// pass leaves mostly rounding error (an exhausted Krylov space).  Otherwise
+// pass left some direction of the block (a combination of its columns, not
+// only a single column) with less than kReorthDrop of the block's largest
+// squared column norm: then one pass leaves mostly rounding error (an
+// exhausted Krylov space).  Otherwise
 // the orthogonality lost in one pass stays near the storage precision times
 // 10^3, and the reported eigenvalues and residuals are explicit FP64 Rayleigh
 // quotients and residual norms in any case.
@@ -359,14 +361,18 @@ template <typename F> void field_to_coef(F const *M, int Bf, double *out) {
 
 // Cholesky G = R^H R of a B x B Hermitian positive definite matrix; returns
 // false if a pivot is not positive (rank deficiency).  R upper, Rinv = R^-1.
-template <typename F> bool cholesky_upper(std::vector<F> const &G, int B, std::vector<F> &R, std::vector<F> &Rinv) {
+// Fails when a pivot (the squared part of a column outside the span of the
+// columns before it) is not above 1e-28 of the largest diagonal entry, or not
+// above `floor_abs`.
+template <typename F>
+bool cholesky_upper(std::vector<F> const &G, int B, std::vector<F> &R, std::vector<F> &Rinv, double floor_abs) {
   std::vector<F> L(static_cast<size_t>(B) * B, F(0.0));
   double scale = 0.0;
   for (int i = 0; i < B; ++i) scale = std::max(scale, Field<F>::re(G[static_cast<size_t>(i) * B + i]));
   for (int j = 0; j < B; ++j) {
     double d = Field<F>::re(G[static_cast<size_t>(j) * B + j]);
     for (int k = 0; k < j; ++k) d -= std::norm(L[static_cast<size_t>(j) * B + k]);
-    if (!(d > 1e-28 * std::max(scale, 1e-300))) return false;
+    if (!(d > std::max(1e-28 * std::max(scale, 1e-300), floor_abs))) return false;
     double const ljj = std::sqrt(d);
     L[static_cast<size_t>(j) * B + j] = ljj;
     for (int i = j + 1; i < B; ++i) {
@@ -621,8 +627,13 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
   // CholQR2 of block X into out (may be X); Rf receives the factor R per
   // segment (X = out R).  A segment whose Gram matrix is singular has run out
   // of Krylov directions (A applied to the last block falls inside the
-  // stored blocks, in some or all directions).  Its block keeps the
-  // directions whose Gram eigenvalues stand above the storage precision
+  // stored blocks, in some or all directions).  Singular also means a
+  // direction below kNoise of w_scale[s], the largest squared column norm of
+  // the product block X came from: so small a direction is rounding noise of
+  // the orthogonalisation, redrawn at its own size by every rounding of the
+  // block (components along the stored blocks included), and normalising it
+  // would carry those components into the basis.  Its block keeps
+  // the directions whose Gram eigenvalues stand above the storage precision
   // (normalised), gets fresh random columns in place of the others,
   // orthogonalised twice against the nprev stored blocks at Vbase, and goes
   // through CholQR2 again; R is then formed explicitly as out^H X, so the
@@ -630,8 +641,13 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
   // is filled from explicit projections and needs no change.
   uint64_t refills = 0;
   DeviceArray<double> Xorig; // X before a refill (reals of the block's type)
-  auto cholqr = [&]<typename T>(T *X, T *out, T const *Vbase, int nprev) -> bool {
+  auto cholqr = [&]<typename T>(T *X, T *out, T const *Vbase, int nprev, std::vector<double> const *w_scale) -> bool {
     constexpr double kKeep = std::is_same_v<T, float> ? 1e-10 : 1e-24; // relative Gram eigenvalue kept
+    // squared size, against ||W||^2, below which a direction is rounding
+    // noise of the orthogonalisation: sigma below 1e-4 ||W|| (FP32 storage)
+    // or 1e-8 ||W|| (FP64)
+    constexpr double kNoise = std::is_same_v<T, float> ? 1e-8 : 1e-16;
+    auto floor_of = [&](int s) { return w_scale != nullptr ? kNoise * (*w_scale)[s] : 0.0; };
     std::vector<int> refilled(ns, 0);
     bool saved = false;
     std::vector<double> Cr;
@@ -654,7 +670,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         for (int s = 0; s < ns; ++s) {
           std::vector<F> Gs(Gf.begin() + static_cast<int64_t>(s) * Vf, Gf.begin() + static_cast<int64_t>(s + 1) * Vf);
           std::vector<F> R, Rinv;
-          if (!cholesky_upper(Gs, Bf, R, Rinv)) {
+          if (!cholesky_upper(Gs, Bf, R, Rinv, pass == 0 ? floor_of(s) : 0.0)) {
             // only the first pass sees X itself; the second sees an
             // orthonormalised block and cannot be singular unless X was
             if (pass == 1)
@@ -711,7 +727,8 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         if (!field_eigh(Gs, Bf, lam, U)) return fail("block Lanczos: Gram eigen-decomposition failed");
         double const top = lam[Bf - 1];
         int k = 0;
-        for (int c = Bf - 1; c >= 0 && top > 0.0 && lam[c] > kKeep * top; --c, ++k)
+        double const thr = std::max(kKeep * top, floor_of(s));
+        for (int c = Bf - 1; c >= 0 && top > 0.0 && lam[c] > thr; --c, ++k)
           for (int r = 0; r < Bf; ++r) Ms[r * Bf + k] = U[static_cast<size_t>(r) * Bf + c] / std::sqrt(lam[c]);
         keep[s] = k;
       }
@@ -775,11 +792,13 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         int const nb = j + 1;
         Csum.assign(static_cast<size_t>(ns) * nb * Vf, F(0.0));
         // The second pass only when the first removed nearly all of some
-        // column (squared norm left below kReorthDrop of ||W||^2, by
-        // Pythagoras from ||W||^2 and V^H W), where one pass leaves mostly
-        // rounding error (an exhausted Krylov space, a converged direction).
+        // direction of the block (the smallest eigenvalue of what it leaves
+        // below kReorthDrop of ||W||^2, by Pythagoras from W^H W and V^H W),
+        // where one pass leaves mostly rounding error (an exhausted Krylov
+        // space, a converged direction).
         // Pass 0 also forms W^H W (the last block of its Gram pass).
         bool second = false;
+        std::vector<double> wscale(ns, 0.0); // largest squared column norm of W per segment
         for (int pass = 0; pass < (second ? 2 : 1); ++pass) {
           if (pass == 0) {
             if (!alg.gram(Vbase, blk, nb, Wt, Cw, Wt)) return -1;
@@ -798,14 +817,35 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
           }
           gram_to_field(C, static_cast<size_t>(ns) * nb, Bf, Cf);
           if (pass == 0) {
-            for (int s = 0; s < ns && !second; ++s)
-              for (int q = 0; q < Bf && !second; ++q) {
-                double const before = Field<F>::re(Gwf[static_cast<size_t>(s) * Vf + q * Bf + q]);
-                double proj = 0.0;
-                for (int i = 0; i < nb; ++i)
-                  for (int a = 0; a < Bf; ++a) proj += std::norm(Cf[(static_cast<size_t>(s) * nb + i) * Vf + a * Bf + q]);
-                if (before - proj < kReorthDrop * before) second = true;
+            // what the first pass leaves, X^H X = W^H W - C^H C (Pythagoras);
+            // its smallest eigenvalue against the largest squared column
+            // norm of W: a direction of the block can vanish while every
+            // column keeps part of its norm
+            for (int s = 0; s < ns; ++s) {
+              std::vector<F> Ms(Vf);
+              double wmax = 0.0;
+              for (int p = 0; p < Bf; ++p) {
+                wmax = std::max(wmax, Field<F>::re(Gwf[static_cast<size_t>(s) * Vf + p * Bf + p]));
+                for (int q = 0; q < Bf; ++q) {
+                  F v = Gwf[static_cast<size_t>(s) * Vf + p * Bf + q];
+                  for (int i = 0; i < nb; ++i) {
+                    F const *Ci = Cf.data() + (static_cast<size_t>(s) * nb + i) * Vf;
+                    for (int a = 0; a < Bf; ++a) v -= Field<F>::conj(Ci[a * Bf + p]) * Ci[a * Bf + q];
+                  }
+                  Ms[p * Bf + q] = v;
+                }
               }
+              for (int p = 0; p < Bf; ++p)
+                for (int q = 0; q < p; ++q) {
+                  F const h = 0.5 * (Ms[p * Bf + q] + Field<F>::conj(Ms[q * Bf + p]));
+                  Ms[p * Bf + q] = h;
+                  Ms[q * Bf + p] = Field<F>::conj(h);
+                }
+              wscale[s] = wmax;
+              std::vector<double> lam;
+              std::vector<F> U;
+              if (!field_eigh(Ms, Bf, lam, U) || lam[0] < kReorthDrop * wmax) second = true;
+            }
           }
           for (size_t t = 0; t < Cf.size(); ++t) {
             Csum[t] += Cf[t];
@@ -833,9 +873,9 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
           }
         }
         if (j + 1 < m) {
-          if (!cholqr(Wt, Vj(j + 1), Vbase, nb)) return -1;
+          if (!cholqr(Wt, Vj(j + 1), Vbase, nb, &wscale)) return -1;
         } else {
-          if (!cholqr(Wt, Wt, Vbase, nb)) return -1; // residual block; Rf holds its factor
+          if (!cholqr(Wt, Wt, Vbase, nb, &wscale)) return -1; // residual block; Rf holds its factor
         }
       }
       // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
@@ -901,7 +941,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
     float *V32 = reinterpret_cast<float *>(Vb.ptr);
     normal_kernel<float><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, V32, blk);
     LSAMD_HIP(hipGetLastError());
-    if (!cholqr(V32, V32, V32, 0)) return false;
+    if (!cholqr(V32, V32, V32, 0, nullptr)) return false;
     status = run_cycles(V32, reinterpret_cast<float *>(W.ptr), reinterpret_cast<float *>(Y.ptr));
     if (status < 0) return false;
     if (status == 1) {
@@ -912,21 +952,26 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
       LSAMD_HIP(hipMemcpyAsync(Vb.ptr, Y.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
       for (int s = 0; s < ns; ++s) std::fill(Tp[s].begin(), Tp[s].end(), F(0.0));
       kept = 0;
-      if (!cholqr(Vb.ptr, Vb.ptr, Vb.ptr, 0)) return false;
+      if (!cholqr(Vb.ptr, Vb.ptr, Vb.ptr, 0, nullptr)) return false;
     }
   } else {
     normal_kernel<double><<<grid_for(blk, kThreads), kThreads, 0, stream()>>>(o.seed, Vb.ptr, blk);
     LSAMD_HIP(hipGetLastError());
-    if (!cholqr(Vb.ptr, Vb.ptr, Vb.ptr, 0)) return false;
+    if (!cholqr(Vb.ptr, Vb.ptr, Vb.ptr, 0, nullptr)) return false;
     status = 1;
   }
   if (status == 1 && run_cycles(Vb.ptr, W.ptr, Y.ptr) < 0) return false;
   // Final: the lowest Ritz block (in Y), its FP64 Rayleigh quotients and residuals.
   if (!apply(false, Y.ptr, W.ptr)) return false;
-  std::vector<double> Gr;
-  std::vector<F> Th, Grf;
+  std::vector<double> Gr, Gy;
+  std::vector<F> Th, Grf, Gyf;
   if (!alg.gram(Y.ptr, blk, 1, W.ptr, G)) return false; // Y^H A Y
   gram_to_field(G, static_cast<size_t>(ns), Bf, Th);
+  // Y^H Y: the quotients and residuals are taken per unit column, so a Ritz
+  // block that is orthonormal only to the solver's accuracy still gives
+  // Rayleigh quotients accurate to the square of its residuals
+  if (!alg.gram(Y.ptr, blk, 1, Y.ptr, Gy)) return false;
+  gram_to_field(Gy, static_cast<size_t>(ns), Bf, Gyf);
   std::vector<F> negTh(Th.size());
   for (size_t t = 0; t < Th.size(); ++t) negTh[t] = -Th[t];
   set_coef(negTh, 1);
@@ -940,8 +985,9 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
       double lam = 0.0, r2 = 0.0;
       for (int c = 0; c < multiplicity; ++c) {
         int const q = multiplicity * e + c;
-        lam += Field<F>::re(Th[static_cast<size_t>(s) * Vf + q * Bf + q]);
-        r2 = std::max(r2, Field<F>::re(Grf[static_cast<size_t>(s) * Vf + q * Bf + q]));
+        double const n2 = Field<F>::re(Gyf[static_cast<size_t>(s) * Vf + q * Bf + q]);
+        lam += Field<F>::re(Th[static_cast<size_t>(s) * Vf + q * Bf + q]) / n2;
+        r2 = std::max(r2, Field<F>::re(Grf[static_cast<size_t>(s) * Vf + q * Bf + q]) / n2);
       }
       lam /= multiplicity;
       double const rel = std::sqrt(std::max(0.0, r2)) / std::max(1.0, std::fabs(lam));
@@ -949,13 +995,15 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
       res.residuals[static_cast<size_t>(listed[s]) * o.nev + e] = rel;
       rmax = std::max(rmax, rel);
     }
-  res.first_rayleigh = Field<F>::re(Th[0]);
-  res.first_residual = std::sqrt(std::max(0.0, Field<F>::re(Grf[0])));
+  res.first_rayleigh = Field<F>::re(Th[0]) / Field<F>::re(Gyf[0]);
+  res.first_residual = std::sqrt(std::max(0.0, Field<F>::re(Grf[0]) / Field<F>::re(Gyf[0])));
   if (d_ritz_block != nullptr)
     LSAMD_HIP(hipMemcpyAsync(d_ritz_block, Y.ptr, sizeof(double) * blk, hipMemcpyDeviceToDevice, stream()));
   LSAMD_HIP(hipStreamSynchronize(stream()));
   res.cycles = cycles_run;
-  res.converged = done;
+  // the bounds decide when to stop; convergence is also claimed only when
+  // the explicit residuals agree
+  res.converged = done && rmax <= std::max(10.0 * o.tol, 1e-12);
   res.fp64_cycles = fp64_cycles;
   res.blocks = blocks_done;
   res.products = prods;

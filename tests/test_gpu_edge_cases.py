@@ -192,3 +192,23 @@ def test_block_solvers_survive_an_exhausted_krylov_space(engine):
     assert r["converged"]
     for s, (_, M) in enumerate(ref):
         assert np.abs(r["eigenvalues"][s] - np.linalg.eigvalsh(M)[:2]).max() < 1e-10, (sl[s], r["eigenvalues"][s])
+
+
+def test_batched_solver_keeps_its_basis_orthonormal_when_a_direction_vanishes(engine):
+    """The same exhausted sectors from twelve start blocks.  A Krylov block can
+    lose one direction (a combination of its columns) while every column keeps
+    part of its norm; the step must then orthogonalise twice, and a direction
+    left at rounding-noise size must be replaced, not normalised, or the basis
+    stops being orthonormal and Ritz values leave the spectrum."""
+    n, h = 16, 8
+    expr = zz_chain(n)
+    sl = [(0, None), (3, None), (8, None)]
+    bases, ops = LS.chain_sector_operators(n, h, expr, sl)
+    ref = [np.linalg.eigvalsh(M)[:2] for _, M in H.chain_sector_matrices(n, h, expr, sl, LS.DEFAULT_NORM_TOLERANCE)]
+    batch = lsamd.DeviceSectorBatch(ops)
+    for seed in range(1, 13):
+        r = batch.eigensolve(nev=2, block_size=4, max_blocks=12, keep_blocks=3, max_cycles=10, tol=1e-8, seed=seed)
+        say("exhausted sectors, seed", seed, r["cycles"], r["max_residual"])
+        assert r["converged"] and r["max_residual"] <= 1e-7, (seed, r)
+        for s in range(len(sl)):
+            assert np.abs(r["eigenvalues"][s] - ref[s]).max() < 1e-10, (seed, sl[s], r["eigenvalues"][s])
