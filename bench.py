@@ -219,7 +219,7 @@ def sectors_bench(args):
                      **traffic_rate(traffic, ms),
                      "kernel": sector_kernel, "kernel_ms": ms, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "bytes_model": "compulsory: CSR stream once, x read once, y written once",
-                     "gather_model_GBs": gather_model},
+                     "gather_model_GBs": gather_model, "gather_model_frac": gather_model / HBM_PEAK_GBS},
         "setup_seconds": t_setup,
         "eigensolve": {"method": "complex thick-restart block Lanczos, FP32 then FP64",
                        "block_complex_columns": 4, "schedule": sched4, "max_blocks": args.sector_blocks,
@@ -885,7 +885,10 @@ def main():
         "fp32_block8": {"nonzero_rhs_per_s": block_rate, "speedup_vs_fp64": block_rate / value,
                         "kernel_ms": b_ms_mean, "achieved_GBs": block_gbs, "frac": block_gbs / HBM_PEAK_GBS,
                         "bytes_model": "compulsory: CSR stream once, x read once, y written once",
-                        "gather_model_GBs": block_gather_gbs},
+                        "gather_model_GBs": block_gather_gbs,
+                        "gather_model_frac": block_gather_gbs / HBM_PEAK_GBS,
+                        "gather_model": "per stored nonzero 8 B (column, value) + one 32 B row of x; "
+                                        "per row 8 + 4 + 64 B"},
         "general_case": general,
         "otf_fp64": {"seconds": t_otf, "nonzeros_per_s": nnz / t_otf},
         "hdf5_io": h5,
