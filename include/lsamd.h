@@ -425,15 +425,19 @@ typedef struct ls_amd_eig_info {
   int cycles;            /* restart cycles run */
   int fp64_cycles;       /* of which with the FP64 product */
   int blocks;            /* block products in the Krylov cycles */
-  int converged;         /* 1 if every segment met tol */
+  int converged;         /* 1 if every segment met tol: the residual bounds
+                            fell below tol and the explicit final residuals
+                            are within 10 tol */
   int64_t products;      /* all block products, the final check included */
   double product_seconds;
   double total_seconds;
   double max_residual;   /* largest FP64 relative residual reported */
 } ls_amd_eig_info;
 
-/* Thick-restart block Lanczos on one operator (FP64 Krylov blocks and
-   orthogonalisation, two passes of full re-orthogonalisation, CholQR2).
+/* Thick-restart block Lanczos on one operator (Krylov blocks stored in the
+   product's precision, FP64 orthogonalisation arithmetic: full
+   re-orthogonalisation, a second pass when some direction of the block
+   nearly vanishes, CholQR2, fresh random directions for exhausted ones).
    block_size in {1, 2, 4, 8}; nev <= block_size eigenvalues; max_blocks
    Krylov blocks per cycle; keep_blocks Ritz blocks kept at a restart.
    mode 1: FP32 block product until the relative residual falls below
