@@ -853,6 +853,18 @@ def main():
         log(f"E0 fp64 {r64['eigenvalues'][0]:.15f} ({r64['steps']} steps, {r64['total_seconds']:.2f} s), "
             f"fp32 block of {args.block_size} -> fp64 {e_b:.15f} ({rb['cycles']} cycles, {rb['fp64_cycles']} fp64, "
             f"{rb['products']} products, {rb['total_seconds']:.2f} s), |dE0/E0| = {d:.2e}")
+        # the same solve on the FP32 block-of-8 product that fp32_block8 times
+        r8 = op.block_eigensolve("fp32", block_size=8, nev=1, max_blocks=args.block_cycle,
+                                 keep_blocks=args.block_keep, max_cycles=40, tol=1e-8, switch_tol=1e-5)
+        e_8 = float(r8["eigenvalues"][0])
+        d8 = abs(e_8 - r64["eigenvalues"][0]) / abs(r64["eigenvalues"][0])
+        e0["block8"] = {"E0_fp32_block_then_fp64": e_8, "dE0_rel": d8, "block_size": 8,
+                        "block_cycle": args.block_cycle, "block_keep": args.block_keep, "cycles": r8["cycles"],
+                        "fp64_cycles": r8["fp64_cycles"], "products": r8["products"],
+                        "relative_residual": float(r8["residuals"][0]), "seconds": r8["total_seconds"],
+                        "product_seconds": r8["product_seconds"]}
+        log(f"fp32 block of 8 -> fp64 {e_8:.15f} ({r8['cycles']} cycles, {r8['fp64_cycles']} fp64, "
+            f"{r8['products']} products, {r8['total_seconds']:.2f} s), |dE0/E0| = {d8:.2e}")
 
     line = {
         "metric": "H·x nonzeros/s + basis states/s, 6×6 J1-J2; FP32-block vs FP64 at ΔE0≤1e-10",
