@@ -645,8 +645,10 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
     constexpr double kKeep = std::is_same_v<T, float> ? 1e-10 : 1e-24; // relative Gram eigenvalue kept
     // squared size, against ||W||^2, below which a direction is rounding
     // noise of the orthogonalisation: sigma below 1e-4 ||W|| (FP32 storage)
-    // or 1e-8 ||W|| (FP64)
-    constexpr double kNoise = std::is_same_v<T, float> ? 1e-8 : 1e-16;
+    // or min(1e-8, tol / 10) ||W|| (FP64: a refill drops what it replaces
+    // from the Krylov relation, so it must stay below the residuals asked for)
+    double const noise64 = std::min(1e-8, 0.1 * o.tol);
+    double const kNoise = std::is_same_v<T, float> ? 1e-8 : noise64 * noise64;
     auto floor_of = [&](int s) { return w_scale != nullptr ? kNoise * (*w_scale)[s] : 0.0; };
     std::vector<int> refilled(ns, 0);
     bool saved = false;

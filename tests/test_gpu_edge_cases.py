@@ -212,3 +212,8 @@ def test_batched_solver_keeps_its_basis_orthonormal_when_a_direction_vanishes(en
         assert r["converged"] and r["max_residual"] <= 1e-7, (seed, r)
         for s in range(len(sl)):
             assert np.abs(r["eigenvalues"][s] - ref[s]).max() < 1e-10, (seed, sl[s], r["eigenvalues"][s])
+    # a tolerance below the FP64 refill limit of 1e-8 ||W||: the limit follows it
+    for seed in (1, 2, 3):
+        r = batch.eigensolve(nev=2, block_size=4, max_blocks=12, keep_blocks=3, max_cycles=20, tol=1e-10, seed=seed)
+        say("exhausted sectors, tol 1e-10, seed", seed, r["cycles"], r["max_residual"])
+        assert r["converged"] and r["max_residual"] <= 1e-9, (seed, r)
