@@ -522,12 +522,21 @@ __global__ __launch_bounds__(256) void csr_compact_kernel(int64_t n, int64_t con
 constexpr int kKeyLocalBits = 24; // entries per source row < 2^24 (one per term at most)
 constexpr int kSortCap = 1024;    // longest row sorted in LDS; longer rows are heap-sorted in place
 
+// The count and scatter passes give kPushLanes lanes to a source row: lane l
+// takes its entries b + l, b + l + kPushLanes, ..., so a wave reads the
+// column stream coalesced and has a row's atomics in flight together.  Slot
+// order within a target row follows the atomics, and the sort that comes
+// next fixes it.
+constexpr int kPushLanes = 16;
+
 __global__ __launch_bounds__(256) void push_count_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                          uint32_t const *__restrict__ cols,
                                                          uint32_t *__restrict__ counts) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kPushLanes;
+  int const l = static_cast<int>(threadIdx.x) % kPushLanes;
   if (i >= n) return;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) atomicAdd(&counts[cols[p]], 1u);
+  int64_t const e = rp[i + 1];
+  for (int64_t p = rp[i] + l; p < e; p += kPushLanes) atomicAdd(&counts[cols[p]], 1u);
 }
 
 __global__ __launch_bounds__(256) void push_scatter_kernel(int64_t n, int64_t const *__restrict__ rp,
@@ -535,10 +544,11 @@ __global__ __launch_bounds__(256) void push_scatter_kernel(int64_t n, int64_t co
                                                            int64_t const *__restrict__ target_ptr,
                                                            uint32_t *__restrict__ cursor,
                                                            uint64_t *__restrict__ keys) {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kPushLanes;
+  int const l = static_cast<int>(threadIdx.x) % kPushLanes;
   if (i >= n) return;
   int64_t const b = rp[i], e = rp[i + 1];
-  for (int64_t p = b; p < e; ++p) {
+  for (int64_t p = b + l; p < e; p += kPushLanes) {
     uint32_t const j = cols[p];
     uint32_t const slot = atomicAdd(&cursor[j], 1u);
     keys[target_ptr[j] + slot] = (static_cast<uint64_t>(i) << kKeyLocalBits) | static_cast<uint64_t>(p - b);
@@ -631,7 +641,7 @@ bool transpose_to_targets(int64_t n, int64_t nnz, int width, DeviceArray<int64_t
       !keys.allocate(static_cast<size_t>(nnz)))
     return false;
   LSAMD_HIP(hipMemsetAsync(counts.ptr, 0, sizeof(uint32_t) * n, stream()));
-  push_count_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, counts.ptr);
+  push_count_kernel<<<grid_for(n * kPushLanes, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, counts.ptr);
   LSAMD_HIP(hipGetLastError());
   uint64_t total = 0;
   if (!exclusive_scan(counts.ptr, n, reinterpret_cast<uint64_t *>(target_ptr.ptr), total)) return false;
@@ -639,8 +649,8 @@ bool transpose_to_targets(int64_t n, int64_t nnz, int width, DeviceArray<int64_t
   int64_t const last = nnz;
   LSAMD_HIP(hipMemcpyAsync(target_ptr.ptr + n, &last, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
   LSAMD_HIP(hipMemsetAsync(counts.ptr, 0, sizeof(uint32_t) * n, stream()));
-  push_scatter_kernel<<<grid_for(n, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, target_ptr.ptr,
-                                                              counts.ptr, keys.ptr);
+  push_scatter_kernel<<<grid_for(n * kPushLanes, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr,
+                                                                           target_ptr.ptr, counts.ptr, keys.ptr);
   LSAMD_HIP(hipGetLastError());
   int const sort_grid = static_cast<int>(std::min<int64_t>(n, 1 << 16));
   sort_rows_kernel<<<sort_grid, 64, 0, stream()>>>(n, target_ptr.ptr, keys.ptr);
