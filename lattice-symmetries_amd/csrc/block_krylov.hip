@@ -632,13 +632,14 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
   // the product block X came from: so small a direction is rounding noise of
   // the orthogonalisation, redrawn at its own size by every rounding of the
   // block (components along the stored blocks included), and normalising it
-  // would carry those components into the basis.  Its block keeps
-  // the directions whose Gram eigenvalues stand above the storage precision
-  // (normalised), gets fresh random columns in place of the others,
-  // orthogonalised twice against the nprev stored blocks at Vbase, and goes
-  // through CholQR2 again; R is then formed explicitly as out^H X, so the
-  // Krylov relation and the residual bounds stay exact.  The projected matrix
-  // is filled from explicit projections and needs no change.
+  // would carry those components into the basis.  Such a block keeps the
+  // directions whose Gram eigenvalues stand above kKeep of the largest and
+  // above the noise limit (normalised), gets fresh random columns in place of
+  // the others, orthogonalised twice against the nprev stored blocks at
+  // Vbase, and goes through CholQR2 again; R is then formed explicitly as
+  // out^H X, so the Krylov relation and the residual bounds hold up to the
+  // directions replaced, which lie below the noise limit.  The projected
+  // matrix is filled from explicit projections and needs no change.
   uint64_t refills = 0;
   DeviceArray<double> Xorig; // X before a refill (reals of the block's type)
   auto cholqr = [&]<typename T>(T *X, T *out, T const *Vbase, int nprev, std::vector<double> const *w_scale) -> bool {
