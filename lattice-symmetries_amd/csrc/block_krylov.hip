@@ -767,17 +767,58 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
   bool done = false;
   std::vector<double> C, Cw, Gw;
   std::vector<F> Cf, Csum, Mf, Gwf;
-  // coefficients of Ritz block t: M[s][i] = Z_s[i-block rows, t-block columns]
-  auto ritz_coef = [&](int t) {
-    Mf.assign(static_cast<size_t>(ns) * m * Vf, F(0.0));
+  // Rayleigh-Ritz on the leading nbk blocks of each segment's projected
+  // matrix: theta, Z (nbk Bf square) and the residual bounds || Rf Z_last ||,
+  // Rf being the factor of block nbk (the one after the last used).
+  int zdim = dimT; // order of the last Rayleigh-Ritz problem
+  auto rayleigh_ritz = [&](int nbk) -> bool {
+    int const d = nbk * Bf;
+    zdim = d;
+    std::vector<int> eig_ok(ns, 1);
+    parallel_for(ns, [&](int s) {
+      if (d == dimT) {
+        eig_ok[s] = field_eigh(Tp[s], dimT, theta[s], Z[s]) ? 1 : 0;
+      } else {
+        std::vector<F> Ts(static_cast<size_t>(d) * d);
+        for (int r = 0; r < d; ++r)
+          std::copy(Tp[s].begin() + static_cast<int64_t>(r) * dimT, Tp[s].begin() + static_cast<int64_t>(r) * dimT + d,
+                    Ts.begin() + static_cast<int64_t>(r) * d);
+        eig_ok[s] = field_eigh(Ts, d, theta[s], Z[s]) ? 1 : 0;
+      }
+      if (!eig_ok[s]) return;
+      double w = 0.0;
+      F const *Rs = Rf.data() + static_cast<size_t>(s) * Vf;
+      for (int q = 0; q < want; ++q) {
+        double rs = 0.0;
+        for (int p = 0; p < Bf; ++p) {
+          F acc = 0.0;
+          for (int c = 0; c < Bf; ++c) acc += Rs[p * Bf + c] * Z[s][static_cast<size_t>((nbk - 1) * Bf + c) * d + q];
+          rs += std::norm(acc);
+        }
+        w = std::max(w, std::sqrt(rs) / std::max(1.0, std::fabs(theta[s][q])));
+      }
+      worst[s] = w;
+    });
     for (int s = 0; s < ns; ++s)
-      for (int i = 0; i < m; ++i)
+      if (!eig_ok[s]) return fail("projected eigenproblem did not converge in segment " + std::to_string(listed[s]));
+    return true;
+  };
+  // coefficients of Ritz block t over nbk blocks: M[s][i] = Z_s[i-block rows, t-block columns]
+  auto ritz_coef = [&](int t, int nbk) {
+    Mf.assign(static_cast<size_t>(ns) * nbk * Vf, F(0.0));
+    for (int s = 0; s < ns; ++s)
+      for (int i = 0; i < nbk; ++i)
         for (int p = 0; p < Bf; ++p)
           for (int q = 0; q < Bf; ++q)
-            Mf[(static_cast<size_t>(s) * m + i) * Vf + p * Bf + q] =
-                Z[s][static_cast<size_t>(i * Bf + p) * dimT + t * Bf + q];
-    set_coef(Mf, m);
+            Mf[(static_cast<size_t>(s) * nbk + i) * Vf + p * Bf + q] =
+                Z[s][static_cast<size_t>(i * Bf + p) * zdim + t * Bf + q];
+    set_coef(Mf, nbk);
   };
+  // In the FP64 cycles a trial Rayleigh-Ritz after each block lets the solve
+  // stop inside a cycle, where the host work is small beside a product
+  // (segments x order^3 of the projected matrix at most kTrialWork).
+  constexpr double kTrialWork = 4e6;
+  bool const trials = static_cast<double>(ns) * dimT * dimT * dimT <= kTrialWork;
 
   // Restart cycles with the blocks stored as T: float runs the FP32 product,
   // double the FP64 product.  `kept` blocks (and their part of Tp) are in
@@ -877,32 +918,26 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
         }
         if (j + 1 < m) {
           if (!cholqr(Wt, Vj(j + 1), Vbase, nb, &wscale)) return -1;
+          if (!fp32 && trials && j + 1 - kept >= 2) {
+            if (!rayleigh_ritz(j + 1)) return -1;
+            double wmax = 0.0;
+            for (int s = 0; s < ns; ++s) wmax = std::max(wmax, worst[s]);
+            if (wmax < o.tol) {
+              ++cycles_run;
+              res.history.push_back(wmax);
+              done = true;
+              ritz_coef(0, j + 1);
+              if (!alg.combine(Y.ptr, 0.0, Vbase, blk, j + 1, coef)) return -1;
+              ++cycle;
+              return 0;
+            }
+          }
         } else {
           if (!cholqr(Wt, Wt, Vbase, nb, &wscale)) return -1; // residual block; Rf holds its factor
         }
       }
       // Rayleigh-Ritz per segment, residual bounds || Rf Z_last ||
-      std::vector<int> eig_ok(ns, 1);
-      parallel_for(ns, [&](int s) {
-        eig_ok[s] = field_eigh(Tp[s], dimT, theta[s], Z[s]) ? 1 : 0;
-        if (!eig_ok[s]) return;
-        double w = 0.0;
-        F const *Rs = Rf.data() + static_cast<size_t>(s) * Vf;
-        for (int q = 0; q < want; ++q) {
-          double rs = 0.0;
-          for (int p = 0; p < Bf; ++p) {
-            F acc = 0.0;
-            for (int c = 0; c < Bf; ++c)
-              acc += Rs[p * Bf + c] * Z[s][static_cast<size_t>((m - 1) * Bf + c) * dimT + q];
-            rs += std::norm(acc);
-          }
-          w = std::max(w, std::sqrt(rs) / std::max(1.0, std::fabs(theta[s][q])));
-        }
-        worst[s] = w;
-      });
-      for (int s = 0; s < ns; ++s)
-        if (!eig_ok[s])
-          return fail("projected eigenproblem did not converge in segment " + std::to_string(listed[s])) ? 0 : -1;
+      if (!rayleigh_ritz(m)) return -1;
       double wmax = 0.0;
       for (int s = 0; s < ns; ++s) wmax = std::max(wmax, worst[s]);
       ++cycles_run;
@@ -913,7 +948,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
       bool const to_fp64 = fp32 && !last && (wmax < o.switch_tol || wmax > 0.9 * prev_worst);
       prev_worst = wmax;
       if (done || last || to_fp64) {
-        ritz_coef(0);
+        ritz_coef(0, m);
         if (!alg.combine(Y.ptr, 0.0, Vbase, blk, m, coef)) return -1;
         ++cycle;
         return to_fp64 ? 1 : 0;
@@ -924,7 +959,7 @@ bool krylov_solve(std::vector<int64_t> const &seg_rows, std::vector<char> const 
       // not memory, for B = 8.)
       int const r = o.keep;
       for (int t = 0; t < r; ++t) {
-        ritz_coef(t);
+        ritz_coef(t, m);
         if (!alg.combine(Yt + static_cast<int64_t>(t) * blk, 0.0, Vbase, blk, m, coef)) return -1;
       }
       LSAMD_HIP(hipMemcpyAsync(Vbase, Yt, sizeof(T) * blk * r, hipMemcpyDeviceToDevice, stream()));
