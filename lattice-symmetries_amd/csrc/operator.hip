@@ -539,11 +539,15 @@ __global__ __launch_bounds__(256) void push_count_kernel(int64_t n, int64_t cons
   for (int64_t p = rp[i] + l; p < e; p += kPushLanes) atomicAdd(&counts[cols[p]], 1u);
 }
 
+// Sort keys of the transpose: the source row in the high bits (the column of
+// M, the order within a target row) and, below it, the entry's global
+// position when every position fits 32 bits (wide keys), else its position
+// within the source row (kKeyLocalBits).  Both sort a row the same way.
 __global__ __launch_bounds__(256) void push_scatter_kernel(int64_t n, int64_t const *__restrict__ rp,
                                                            uint32_t const *__restrict__ cols,
                                                            int64_t const *__restrict__ target_ptr,
                                                            uint32_t *__restrict__ cursor,
-                                                           uint64_t *__restrict__ keys) {
+                                                           uint64_t *__restrict__ keys, int wide_keys) {
   int64_t const i = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kPushLanes;
   int const l = static_cast<int>(threadIdx.x) % kPushLanes;
   if (i >= n) return;
@@ -551,7 +555,8 @@ __global__ __launch_bounds__(256) void push_scatter_kernel(int64_t n, int64_t co
   for (int64_t p = b + l; p < e; p += kPushLanes) {
     uint32_t const j = cols[p];
     uint32_t const slot = atomicAdd(&cursor[j], 1u);
-    keys[target_ptr[j] + slot] = (static_cast<uint64_t>(i) << kKeyLocalBits) | static_cast<uint64_t>(p - b);
+    keys[target_ptr[j] + slot] = wide_keys ? (static_cast<uint64_t>(i) << 32) | static_cast<uint64_t>(p)
+                                           : (static_cast<uint64_t>(i) << kKeyLocalBits) | static_cast<uint64_t>(p - b);
   }
 }
 
@@ -619,12 +624,18 @@ __global__ __launch_bounds__(256) void push_finalize_kernel(int64_t nnz, uint64_
                                                             int64_t const *__restrict__ src_ptr,
                                                             double const *__restrict__ src_vals, int width,
                                                             uint32_t *__restrict__ cols,
-                                                            double *__restrict__ vals) {
+                                                            double *__restrict__ vals, int wide_keys) {
   int64_t const q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= nnz) return;
   uint64_t const key = keys[q];
-  int64_t const i = static_cast<int64_t>(key >> kKeyLocalBits);
-  int64_t const p = src_ptr[i] + static_cast<int64_t>(key & ((uint64_t{1} << kKeyLocalBits) - 1));
+  int64_t i, p;
+  if (wide_keys) {
+    i = static_cast<int64_t>(key >> 32);
+    p = static_cast<int64_t>(key & 0xffffffffull);
+  } else {
+    i = static_cast<int64_t>(key >> kKeyLocalBits);
+    p = src_ptr[i] + static_cast<int64_t>(key & ((uint64_t{1} << kKeyLocalBits) - 1));
+  }
   cols[q] = static_cast<uint32_t>(i);
   for (int w = 0; w < width; ++w) vals[q * width + w] = src_vals[p * width + w];
 }
@@ -649,8 +660,10 @@ bool transpose_to_targets(int64_t n, int64_t nnz, int width, DeviceArray<int64_t
   int64_t const last = nnz;
   LSAMD_HIP(hipMemcpyAsync(target_ptr.ptr + n, &last, sizeof(int64_t), hipMemcpyHostToDevice, stream()));
   LSAMD_HIP(hipMemsetAsync(counts.ptr, 0, sizeof(uint32_t) * n, stream()));
+  int const wide_keys = nnz <= int64_t{0xffffffff} ? 1 : 0;
   push_scatter_kernel<<<grid_for(n * kPushLanes, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr,
-                                                                           target_ptr.ptr, counts.ptr, keys.ptr);
+                                                                           target_ptr.ptr, counts.ptr, keys.ptr,
+                                                                           wide_keys);
   LSAMD_HIP(hipGetLastError());
   int const sort_grid = static_cast<int>(std::min<int64_t>(n, 1 << 16));
   sort_rows_kernel<<<sort_grid, 64, 0, stream()>>>(n, target_ptr.ptr, keys.ptr);
@@ -662,7 +675,7 @@ bool transpose_to_targets(int64_t n, int64_t nnz, int width, DeviceArray<int64_t
   DeviceArray<double> vals2;
   if (!cols2.allocate(static_cast<size_t>(nnz)) || !vals2.allocate(static_cast<size_t>(nnz) * width)) return false;
   push_finalize_kernel<<<grid_for(nnz, 256), 256, 0, stream()>>>(nnz, keys.ptr, row_ptr.ptr, vals.ptr, width,
-                                                                 cols2.ptr, vals2.ptr);
+                                                                 cols2.ptr, vals2.ptr, wide_keys);
   LSAMD_HIP(hipGetLastError());
   LSAMD_HIP(hipStreamSynchronize(stream()));
   row_ptr = std::move(target_ptr);
