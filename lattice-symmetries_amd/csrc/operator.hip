@@ -582,17 +582,31 @@ __device__ void heap_sort(uint64_t *a, int64_t n) {
   }
 }
 
-// One wave per row: bitonic sort of the row's keys in LDS, padded to a power
-// of two with all-ones keys.  The row index is uniform over the workgroup,
-// so every barrier is reached by all its threads.
+// Largest entry of counts[0, n) into *out (which starts at zero).
+__global__ __launch_bounds__(256) void max_count_kernel(uint32_t const *__restrict__ counts, int64_t n,
+                                                        uint32_t *__restrict__ out) {
+  uint32_t m = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    m = max(m, counts[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), off, 64)));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// One wave per row: bitonic sort of the row's keys in LDS (dynamic, `cap`
+// keys: the longest row's power of two, at most kSortCap, so that more
+// workgroups fit a CU), padded to a power of two with all-ones keys.  Longer
+// rows are heap-sorted in place.  The row index is uniform over the
+// workgroup, so every barrier is reached by all its threads.
 __global__ __launch_bounds__(64) void sort_rows_kernel(int64_t n, int64_t const *__restrict__ rp,
-                                                       uint64_t *__restrict__ keys) {
-  __shared__ uint64_t s[kSortCap];
+                                                       uint64_t *__restrict__ keys, int cap) {
+  extern __shared__ uint64_t s[];
   for (int64_t row = blockIdx.x; row < n; row += gridDim.x) {
     int64_t const b = rp[row];
     int64_t const len = rp[row + 1] - b;
     if (len <= 1) continue;
-    if (len > kSortCap) {
+    if (len > cap) {
       if (threadIdx.x == 0) heap_sort(keys + b, len);
       continue;
     }
@@ -654,6 +668,18 @@ bool transpose_to_targets(int64_t n, int64_t nnz, int width, DeviceArray<int64_t
   LSAMD_HIP(hipMemsetAsync(counts.ptr, 0, sizeof(uint32_t) * n, stream()));
   push_count_kernel<<<grid_for(n * kPushLanes, 256), 256, 0, stream()>>>(n, row_ptr.ptr, cols.ptr, counts.ptr);
   LSAMD_HIP(hipGetLastError());
+  // the longest target row sizes the sort's LDS
+  DeviceArray<uint32_t> longest;
+  if (!longest.allocate(1)) return false;
+  LSAMD_HIP(hipMemsetAsync(longest.ptr, 0, sizeof(uint32_t), stream()));
+  max_count_kernel<<<static_cast<int>(std::min<int64_t>(grid_for(n, 256), 1024)), 256, 0, stream()>>>(counts.ptr, n,
+                                                                                                  longest.ptr);
+  LSAMD_HIP(hipGetLastError());
+  uint32_t max_len = 0;
+  LSAMD_HIP(hipMemcpyAsync(&max_len, longest.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, stream()));
+  LSAMD_HIP(hipStreamSynchronize(stream()));
+  int sort_cap = 2;
+  while (sort_cap < kSortCap && sort_cap < static_cast<int>(std::min<uint32_t>(max_len, kSortCap))) sort_cap <<= 1;
   uint64_t total = 0;
   if (!exclusive_scan(counts.ptr, n, reinterpret_cast<uint64_t *>(target_ptr.ptr), total)) return false;
   if (static_cast<int64_t>(total) != nnz) return fail("CSR transpose: entry count mismatch");
@@ -666,7 +692,7 @@ bool transpose_to_targets(int64_t n, int64_t nnz, int width, DeviceArray<int64_t
                                                                            wide_keys);
   LSAMD_HIP(hipGetLastError());
   int const sort_grid = static_cast<int>(std::min<int64_t>(n, 1 << 16));
-  sort_rows_kernel<<<sort_grid, 64, 0, stream()>>>(n, target_ptr.ptr, keys.ptr);
+  sort_rows_kernel<<<sort_grid, 64, sizeof(uint64_t) * sort_cap, stream()>>>(n, target_ptr.ptr, keys.ptr, sort_cap);
   LSAMD_HIP(hipGetLastError());
   LSAMD_HIP(hipStreamSynchronize(stream()));
   counts.reset();
